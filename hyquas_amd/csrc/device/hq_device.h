@@ -1,0 +1,143 @@
+// hq_device.h -- internal types of the HIP device layer (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hyquas_amd.h"
+
+namespace hq {
+
+// ---- errors ------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* get_error();
+
+#define HQ_HIP(call)                                                               \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ::hq::set_error(std::string(#call) + ": " + hipGetErrorString(e_));    \
+            return HQ_ERR_HIP;                                                     \
+        }                                                                          \
+    } while (0)
+
+#define HQ_CHECK(cond, code, msg)                                                  \
+    do {                                                                           \
+        if (!(cond)) { ::hq::set_error(msg); return (code); }                      \
+    } while (0)
+
+// ---- complex amplitude -----------------------------------------------------------
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+    R x, y;
+};
+
+// ---- state -------------------------------------------------------------------------
+struct ProfileEntry {
+    int kind;
+    hipEvent_t start, stop;
+};
+
+}  // namespace hq
+
+struct hq_state_s {
+    int n = 0;
+    int precision = HQ_C128;
+    int device = 0;
+    uint64_t amps = 0;
+    size_t amp_bytes = 16;
+    void* data = nullptr;     // current state vector
+    void* buf = nullptr;      // work buffer (TransMM), may be null
+    hipStream_t stream = nullptr;
+    double* d_red = nullptr;  // reduction scratch (device)
+    double* h_red = nullptr;  // reduction scratch (pinned host)
+    size_t red_cap = 0;       // doubles
+    bool profiling = false;
+    std::vector<hq::ProfileEntry> prof;
+    std::vector<hipEvent_t> free_events;
+};
+
+namespace hq {
+
+// Profiling helpers: bracket one launch with events when enabled.
+hq_status prof_begin(hq_state s, int kind, int* slot);
+hq_status prof_end(hq_state s, int slot);
+
+// ---- OShareMem program (device-resident) ---------------------------------------------
+// Op classes of the register kernel.
+enum ShmClass : uint8_t {
+    SHM_SWAP = 0,   // lo <-> hi                       (X, CNOT, CCX)
+    SHM_Y = 1,      // lo' = -i hi, hi' = i lo         (Y, CY)
+    SHM_REAL = 2,   // real 2x2                        (H, RY, CRY, ...)
+    SHM_RX = 3,     // real diagonal, imaginary off-diagonal (RX, CRX)
+    SHM_GEN = 4,    // complex 2x2                     (U2, U3, ...)
+    SHM_DIAG = 5,   // lo *= d0, hi *= d1              (RZ, CRZ, ...)
+    SHM_PHASE = 6,  // hi *= d1                        (T, S, Z, U1, CZ, CU1, ...)
+};
+
+enum ShmWhere : uint8_t { SHM_IN_REG = 0, SHM_IN_THREAD = 1, SHM_IN_BLOCK = 2 };
+
+struct ShmOp {
+    uint8_t cls;
+    uint8_t where;      // ShmWhere of the target
+    uint8_t tpos;       // register-index bit / thread-index bit / block-index bit
+    uint8_t pad0;
+    uint32_t creg;      // control mask over register-index bits
+    uint32_t cthr;      // control mask over thread-index bits
+    uint32_t pad1;
+    uint64_t cblk;      // control mask over block-index bits
+    double m[8];        // coefficients, re/im interleaved, row-major 2x2
+};
+static_assert(sizeof(ShmOp) == 88, "ShmOp layout");
+
+constexpr int SHM_MAX_TILE = 14;
+constexpr int SHM_MAX_TB = 9;
+
+struct ShmStage {
+    uint32_t reg_bits[5];      // tile bit of register-index bit i (RB used)
+    uint32_t thr_bits[SHM_MAX_TB];  // tile bit of thread-index bit i (TB used)
+    uint32_t op_begin, op_end;
+};
+
+struct ShmHeader {
+    uint64_t block_phys[64];   // physical bit (as a mask) of block-index bit j
+    uint64_t tile_phys[SHM_MAX_TILE];  // physical bit (as a mask) of tile bit i
+    uint32_t swz_col[SHM_MAX_TILE];    // LDS swizzle column of tile bit i
+    uint32_t num_block_bits;
+    uint32_t num_stages;
+    uint32_t num_ops;
+    uint32_t pad;
+    // followed by ShmStage[num_stages], ShmOp[num_ops] (8-byte aligned)
+};
+
+}  // namespace hq
+
+struct hq_program_s {
+    int kind = 0;            // 1 sharemem, 2 blas
+    int n = 0;
+    int precision = HQ_C128;
+    // sharemem
+    void* d_prog = nullptr;
+    size_t prog_bytes = 0;
+    int tile = 0, rb = 0, tb = 0, num_stages = 0, num_ops = 0;
+    uint64_t shared_mask = 0;
+    // blas
+    bool identity_perm = true;
+    std::vector<int> perm;
+    void* d_targs = nullptr; // transpose launch arguments (device)
+    void* d_mat = nullptr;   // K x K, column-major, split planes re | im, in state precision
+    int mat_qubits = 0;
+};
+
+namespace hq {
+// Launchers implemented in the .hip files.
+hq_status launch_sharemem(hq_state s, hq_program p);
+hq_status transpose_prepare(hq_state s, const int* perm, void** dev_args);
+hq_status transpose_launch(hq_state s, const void* dev_args, void* in, void* out);
+hq_status launch_dense(hq_state s, const void* d_mat, int mat_qubits, void* in, void* out);
+int sharemem_default_rb(int precision, int tile);
+hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates,
+                                 uint64_t shared_mask, int rb, hq_program* out);
+}  // namespace hq

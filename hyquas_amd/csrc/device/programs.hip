@@ -1,0 +1,186 @@
+// programs.hip -- device-resident group programs and their launches.
+//
+// A program is what Circuit::compile leaves behind for one group: the
+// lowered OShareMem op table (src/executor.cpp:462-493 rebuilt it and copied
+// it to a global __constant__ table on every launch; here it is built once
+// and owned by the program, so launches are re-entrant), or the TransMM
+// transpose plan + fused matrix (src/schedule.cpp:704-783).
+
+#include "hq_device.h"
+
+#include <cmath>
+#include <cstring>
+
+using namespace hq;
+
+extern "C" {
+
+hq_status hq_program_create_sharemem(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask,
+                                     hq_program* out)
+{
+    return build_sharemem_program(s, gates, num_gates, shared_mask, 0, out);
+}
+
+hq_status hq_program_create_blas(hq_state s, const int* perm, const double* mat, int mat_qubits, hq_program* out)
+{
+    HQ_CHECK(s && out, HQ_ERR_ARG, "hq_program_create_blas: null argument");
+    *out = nullptr;
+    HQ_CHECK(mat_qubits >= 0 && mat_qubits <= 7 && mat_qubits <= s->n, HQ_ERR_ARG,
+             "hq_program_create_blas: mat_qubits must be in [0, min(7, n)] (0: transpose only)");
+    HQ_CHECK(mat || !mat_qubits, HQ_ERR_ARG, "hq_program_create_blas: null matrix");
+    const int n = s->n;
+    hq_program p = new hq_program_s();
+    p->kind = 2;
+    p->n = n;
+    p->precision = s->precision;
+    p->mat_qubits = mat_qubits;
+    p->identity_perm = true;
+    if (perm) {
+        uint64_t seen = 0;
+        for (int i = 0; i < n; i++) {
+            if (perm[i] < 0 || perm[i] >= n || ((seen >> perm[i]) & 1)) {
+                delete p;
+                set_error("hq_program_create_blas: perm is not a permutation of 0..n-1");
+                return HQ_ERR_ARG;
+            }
+            seen |= 1ull << perm[i];
+            if (perm[i] != i) p->identity_perm = false;
+        }
+        p->perm.assign(perm, perm + n);
+    }
+    hq_status st = HQ_OK;
+    if (!p->identity_perm) {
+        st = transpose_prepare(s, p->perm.data(), &p->d_targs);
+        if (st != HQ_OK) { hq_program_destroy(p); return st; }
+    }
+    if (mat_qubits == 0) { *out = p; return HQ_OK; }
+    // U as split planes (re | im) in the state's precision: FP64 fusion,
+    // rounded once to FP32 for complex64 states
+    const size_t K = (size_t)1 << mat_qubits;
+    const size_t rbytes = s->precision == HQ_C128 ? 8 : 4;
+    std::vector<char> planes(2 * K * K * rbytes);
+    for (size_t i = 0; i < K * K; i++) {
+        double re = mat[2 * i], im = mat[2 * i + 1];
+        if (!std::isfinite(re) || !std::isfinite(im)) {
+            hq_program_destroy(p);
+            set_error("hq_program_create_blas: matrix entry is not finite");
+            return HQ_ERR_ARG;
+        }
+        if (rbytes == 8) {
+            ((double*)planes.data())[i] = re;
+            ((double*)planes.data())[K * K + i] = im;
+        } else {
+            ((float*)planes.data())[i] = (float)re;
+            ((float*)planes.data())[K * K + i] = (float)im;
+        }
+    }
+    hipError_t e = hipMalloc(&p->d_mat, planes.size());
+    if (e != hipSuccess) {
+        hq_program_destroy(p);
+        set_error(std::string("hq_program_create_blas: hipMalloc: ") + hipGetErrorString(e));
+        return HQ_ERR_OOM;
+    }
+    e = hipMemcpy(p->d_mat, planes.data(), planes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hq_program_destroy(p);
+        set_error(std::string("hq_program_create_blas: hipMemcpy: ") + hipGetErrorString(e));
+        return HQ_ERR_HIP;
+    }
+    *out = p;
+    return HQ_OK;
+}
+
+hq_status hq_program_launch(hq_state s, hq_program p)
+{
+    HQ_CHECK(s && p, HQ_ERR_ARG, "hq_program_launch: null argument");
+    HQ_CHECK(p->n == s->n && p->precision == s->precision, HQ_ERR_ARG,
+             "hq_program_launch: program was built for another qubit count or precision");
+    int slot;
+    hq_status st;
+    if (p->kind == 1) {
+        st = prof_begin(s, 1, &slot);
+        if (st != HQ_OK) return st;
+        st = launch_sharemem(s, p);
+        if (st != HQ_OK) return st;
+        return prof_end(s, slot);
+    }
+    HQ_CHECK(p->kind == 2, HQ_ERR_ARG, "hq_program_launch: unknown program kind");
+    HQ_CHECK(s->buf, HQ_ERR_STATE, "hq_program_launch: TransMM needs a state created with a work buffer");
+    if (!p->identity_perm) {
+        st = prof_begin(s, 2, &slot);
+        if (st != HQ_OK) return st;
+        st = transpose_launch(s, p->d_targs, s->data, s->buf);
+        if (st != HQ_OK) return st;
+        std::swap(s->data, s->buf);
+        st = prof_end(s, slot);
+        if (st != HQ_OK) return st;
+    }
+    if (p->mat_qubits > 0) {
+        st = prof_begin(s, 3, &slot);
+        if (st != HQ_OK) return st;
+        st = launch_dense(s, p->d_mat, p->mat_qubits, s->data, s->buf);
+        if (st != HQ_OK) return st;
+        std::swap(s->data, s->buf);
+        st = prof_end(s, slot);
+        if (st != HQ_OK) return st;
+    }
+    return HQ_OK;
+}
+
+hq_status hq_program_destroy(hq_program p)
+{
+    if (!p) return HQ_OK;
+    if (p->d_prog) (void)hipFree(p->d_prog);
+    if (p->d_targs) (void)hipFree(p->d_targs);
+    if (p->d_mat) (void)hipFree(p->d_mat);
+    delete p;
+    return HQ_OK;
+}
+
+hq_status hq_program_info(hq_program p, int* tile_qubits, int* num_stages, int* num_ops)
+{
+    HQ_CHECK(p, HQ_ERR_ARG, "hq_program_info: null program");
+    if (tile_qubits) *tile_qubits = p->kind == 1 ? p->tile : p->mat_qubits;
+    if (num_stages) *num_stages = p->kind == 1 ? p->num_stages : 0;
+    if (num_ops) *num_ops = p->kind == 1 ? p->num_ops : 0;
+    return HQ_OK;
+}
+
+hq_status hq_apply_sharemem(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask)
+{
+    hq_program p = nullptr;
+    hq_status st = hq_program_create_sharemem(s, gates, num_gates, shared_mask, &p);
+    if (st != HQ_OK) return st;
+    st = hq_program_launch(s, p);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    hq_program_destroy(p);
+    if (st == HQ_OK && e != hipSuccess) { set_error(hipGetErrorString(e)); return HQ_ERR_HIP; }
+    return st;
+}
+
+hq_status hq_apply_transpose(hq_state s, const int* perm)
+{
+    HQ_CHECK(s && perm, HQ_ERR_ARG, "hq_apply_transpose: null argument");
+    hq_program p = nullptr;
+    hq_status st = hq_program_create_blas(s, perm, nullptr, 0, &p);
+    if (st != HQ_OK) return st;
+    st = hq_program_launch(s, p);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    hq_program_destroy(p);
+    if (st == HQ_OK && e != hipSuccess) { set_error(hipGetErrorString(e)); return HQ_ERR_HIP; }
+    return st;
+}
+
+hq_status hq_apply_dense(hq_state s, const double* mat, int mat_qubits)
+{
+    hq_program p = nullptr;
+    hq_status st = hq_program_create_blas(s, nullptr, mat, mat_qubits, &p);
+    if (st != HQ_OK) return st;
+    st = hq_program_launch(s, p);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    hq_program_destroy(p);
+    if (st == HQ_OK && e != hipSuccess) { set_error(hipGetErrorString(e)); return HQ_ERR_HIP; }
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,473 @@
+// sharemem.hip -- OShareMem: one HBM pass applies a whole gate group.
+//
+// Reference behaviour (src/kernelOpt.cu:388-433, src/executor.cpp:462-493,
+// 585-632): each block owns the 2^k amplitudes that differ only in the k
+// "shared" qubits, loads them, applies every gate of the group, stores them.
+// Gates whose target is outside the tile must be diagonal; controls and
+// diagonal targets may sit on block bits.
+//
+// MI355X design (not the reference's): the tile lives in REGISTERS, not in
+// shared memory.  Each thread holds 2^RB amplitudes; the tile bits are split
+// per stage into RB register bits and TB thread bits (k = RB + TB).  Every
+// non-diagonal gate of a stage targets a register bit, so it is a pure
+// register update; diagonal gates and controls on thread or block bits are
+// per-lane / per-block predicates.  Between stages the tile is re-split by
+// one LDS exchange (write, barrier, read).  The first stage reads HBM directly
+// and the last writes HBM directly, so a group with s stages costs one HBM
+// read + write and s-1 LDS round trips instead of one LDS round trip per gate.
+// LDS addresses are XOR-swizzled (tile bits >= 3 fold into bits 0..2) and
+// the host picks lane bits 0..2 so that 8-lane groups hit distinct 16-byte
+// bank groups.
+
+#include "hq_ops.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace hq {
+
+// ---- device side ----------------------------------------------------------------------
+
+template <typename R, int RB, int CLS, int P>
+__device__ __forceinline__ void reg_pairs(cplx<R> (&a)[1 << RB], uint32_t creg, const R* m)
+{
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        if (j & (1 << P)) continue;
+        if ((j & creg) != creg) continue;
+        pair_update<CLS, R>(a[j], a[j | (1 << P)], m);
+    }
+}
+
+template <typename R, int RB, int P>
+__device__ __forceinline__ void reg_op(cplx<R> (&a)[1 << RB], int cls, uint32_t creg, const R* m)
+{
+    switch (cls) {
+    case SHM_SWAP: reg_pairs<R, RB, SHM_SWAP, P>(a, creg, m); break;
+    case SHM_Y: reg_pairs<R, RB, SHM_Y, P>(a, creg, m); break;
+    case SHM_REAL: reg_pairs<R, RB, SHM_REAL, P>(a, creg, m); break;
+    case SHM_RX: reg_pairs<R, RB, SHM_RX, P>(a, creg, m); break;
+    case SHM_GEN: reg_pairs<R, RB, SHM_GEN, P>(a, creg, m); break;
+    case SHM_DIAG: reg_pairs<R, RB, SHM_DIAG, P>(a, creg, m); break;
+    default: reg_pairs<R, RB, SHM_PHASE, P>(a, creg, m); break;
+    }
+}
+
+template <typename R, int RB>
+__device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const ShmOp* __restrict__ op, uint32_t tid,
+                                         uint64_t blk)
+{
+    const uint64_t cblk = op->cblk;
+    if ((blk & cblk) != cblk) return;                 // block control not set (uniform)
+    const uint32_t cthr = op->cthr;
+    if ((tid & cthr) != cthr) return;                 // thread control not set (per lane)
+    R m[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) m[i] = (R)op->m[i];
+    const uint32_t creg = op->creg;
+    const int cls = op->cls;
+    const int tpos = op->tpos;
+    if (op->where == SHM_IN_REG) {
+        switch (tpos) {
+        case 0: reg_op<R, RB, 0>(a, cls, creg, m); break;
+        case 1: if constexpr (RB > 1) reg_op<R, RB, 1>(a, cls, creg, m); break;
+        case 2: if constexpr (RB > 2) reg_op<R, RB, 2>(a, cls, creg, m); break;
+        case 3: if constexpr (RB > 3) reg_op<R, RB, 3>(a, cls, creg, m); break;
+        default: if constexpr (RB > 4) reg_op<R, RB, 4>(a, cls, creg, m); break;
+        }
+        return;
+    }
+    // diagonal gate whose target is a thread bit or a block bit: one factor
+    // per lane for every register amplitude
+    uint32_t hb = op->where == SHM_IN_THREAD ? (tid >> tpos) & 1u : (uint32_t)((blk >> tpos) & 1u);
+    if (cls == SHM_PHASE && !hb) return;
+    const R fr = hb ? m[6] : m[0];
+    const R fi = hb ? m[7] : m[1];
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        if ((j & creg) != creg) continue;
+        scale(a[j], fr, fi);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t swz(uint32_t x, const uint32_t* __restrict__ col)
+{
+    uint32_t f = 0;
+#pragma unroll
+    for (int i = 3; i < K; i++) f ^= ((x >> i) & 1u) ? col[i] : 0u;
+    return x ^ f;
+}
+
+template <typename R, int RB, int TB>
+__global__ __launch_bounds__(1 << TB) void k_sharemem(cplx<R>* __restrict__ psi, const char* __restrict__ prog)
+{
+    constexpr int NR = 1 << RB;
+    constexpr int K = RB + TB;
+    __shared__ cplx<R> lds[1 << K];
+
+    const ShmHeader* __restrict__ H = (const ShmHeader*)prog;
+    const ShmStage* __restrict__ ST = (const ShmStage*)(prog + sizeof(ShmHeader));
+    const uint32_t nst = H->num_stages;
+    const ShmOp* __restrict__ OP = (const ShmOp*)(prog + sizeof(ShmHeader) + nst * sizeof(ShmStage));
+    const uint32_t tid = threadIdx.x;
+    const uint64_t blk = blockIdx.x;
+
+    // block base address: pdep(blk, block bits)
+    uint64_t base = 0;
+    const uint32_t nbb = H->num_block_bits;
+    for (uint32_t j = 0; j < nbb; j++)
+        if ((blk >> j) & 1) base |= H->block_phys[j];
+
+    uint32_t col[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) col[i] = H->swz_col[i];
+
+    cplx<R> a[NR];
+
+    // mapping of stage 0
+    uint32_t rbit[RB];
+    uint64_t rphys[RB];
+#pragma unroll
+    for (int i = 0; i < RB; i++) {
+        rbit[i] = 1u << ST[0].reg_bits[i];
+        rphys[i] = H->tile_phys[ST[0].reg_bits[i]];
+    }
+    uint32_t et = 0;
+    uint64_t pt = 0;
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        uint32_t b = ST[0].thr_bits[i];
+        if ((tid >> i) & 1) { et |= 1u << b; pt |= H->tile_phys[b]; }
+    }
+    const cplx<R>* __restrict__ src = psi + (base | pt);
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        uint64_t off = 0;
+#pragma unroll
+        for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
+        a[j] = src[off];
+    }
+
+    for (uint32_t s = 0; s < nst; s++) {
+        if (s > 0) {
+            // exchange: write under the old mapping, read under the new one
+            const uint32_t st = swz<K>(et, col);
+#pragma unroll
+            for (int j = 0; j < NR; j++) {
+                uint32_t off = 0;
+#pragma unroll
+                for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rbit[i];
+                lds[st ^ swz<K>(off, col)] = a[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < RB; i++) rbit[i] = 1u << ST[s].reg_bits[i];
+            et = 0;
+#pragma unroll
+            for (int i = 0; i < TB; i++)
+                if ((tid >> i) & 1) et |= 1u << ST[s].thr_bits[i];
+            const uint32_t st2 = swz<K>(et, col);
+#pragma unroll
+            for (int j = 0; j < NR; j++) {
+                uint32_t off = 0;
+#pragma unroll
+                for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rbit[i];
+                a[j] = lds[st2 ^ swz<K>(off, col)];
+            }
+        }
+        const uint32_t ob = ST[s].op_begin, oe = ST[s].op_end;
+        for (uint32_t o = ob; o < oe; o++) apply_op<R, RB>(a, OP + o, tid, blk);
+    }
+
+    // store under the mapping of the last stage
+    pt = 0;
+#pragma unroll
+    for (int i = 0; i < TB; i++)
+        if ((et >> ST[nst - 1].thr_bits[i]) & 1) pt |= H->tile_phys[ST[nst - 1].thr_bits[i]];
+#pragma unroll
+    for (int i = 0; i < RB; i++) rphys[i] = H->tile_phys[ST[nst - 1].reg_bits[i]];
+    cplx<R>* __restrict__ dst = psi + (base | pt);
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        uint64_t off = 0;
+#pragma unroll
+        for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
+        dst[off] = a[j];
+    }
+}
+
+// ---- launch table -------------------------------------------------------------------------
+
+using ShmKernel = void (*)(void*, const char*);
+
+template <typename R, int RB, int TB>
+static hq_status launch_variant(hq_state s, hq_program p)
+{
+    uint64_t blocks = (uint64_t)1 << (p->n - p->tile);
+    hipLaunchKernelGGL((k_sharemem<R, RB, TB>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
+                       (cplx<R>*)s->data, (const char*)p->d_prog);
+    HQ_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+template <typename R>
+static hq_status launch_by_shape(hq_state s, hq_program p)
+{
+    const int rb = p->rb, tb = p->tb;
+#define HQ_SHM_CASE(RBV, TBV) \
+    if (rb == RBV && tb == TBV) return launch_variant<R, RBV, TBV>(s, p);
+    HQ_SHM_CASE(1, 6) HQ_SHM_CASE(2, 6) HQ_SHM_CASE(3, 6) HQ_SHM_CASE(4, 6) HQ_SHM_CASE(5, 6)
+    HQ_SHM_CASE(2, 7) HQ_SHM_CASE(3, 7) HQ_SHM_CASE(4, 7) HQ_SHM_CASE(5, 7)
+    HQ_SHM_CASE(3, 8) HQ_SHM_CASE(4, 8) HQ_SHM_CASE(5, 8)
+#undef HQ_SHM_CASE
+    set_error("sharemem: no kernel for this register/thread split");
+    return HQ_ERR_UNSUPPORTED;
+}
+
+static bool shape_supported(int rb, int tb)
+{
+    static const int shapes[][2] = {{1, 6}, {2, 6}, {3, 6}, {4, 6}, {5, 6}, {2, 7}, {3, 7},
+                                    {4, 7}, {5, 7}, {3, 8}, {4, 8}, {5, 8}};
+    for (auto& sh : shapes)
+        if (sh[0] == rb && sh[1] == tb) return true;
+    return false;
+}
+
+hq_status launch_sharemem(hq_state s, hq_program p)
+{
+    return s->precision == HQ_C128 ? launch_by_shape<double>(s, p) : launch_by_shape<float>(s, p);
+}
+
+int sharemem_default_rb(int precision, int tile)
+{
+    // tile = RB + TB; 256-thread blocks with 16 (complex128) or 32 (complex64)
+    // amplitudes per thread at the default 12/13-qubit tiles
+    if (tile <= 10) return std::max(1, tile - 6);
+    if (precision == HQ_C128) return tile >= 13 ? 5 : 4;
+    return tile >= 12 ? 5 : tile - 6;
+}
+
+// ---- host side: lowering + register-stage planning -------------------------------------------
+
+namespace {
+
+struct Lowered {
+    int cls;
+    bool needs_reg;       // non-diagonal update: target must be a register bit
+    int t_tile, t_block;  // target as tile bit or block bit (-1 if not)
+    int c_tile[2], c_block[2];
+    double m[8];
+};
+
+// swizzle column of tile bit i (bits >= 3 fold into the low 3 bits)
+uint32_t swz_column(int i)
+{
+    static const uint32_t cyc[7] = {3, 5, 6, 7, 1, 2, 4};
+    return i < 3 ? (1u << i) : cyc[(i - 3) % 7];
+}
+
+// GF(2) rank of a set of 3-bit vectors
+int rank3(const std::vector<uint32_t>& v)
+{
+    uint32_t basis[3] = {0, 0, 0};
+    int r = 0;
+    for (uint32_t x : v) {
+        for (int b = 2; b >= 0; b--) {
+            if (!((x >> b) & 1)) continue;
+            if (!basis[b]) { basis[b] = x; r++; break; }
+            x ^= basis[b];
+        }
+    }
+    return r;
+}
+
+}  // namespace
+
+hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
+                                 hq_program* out)
+{
+    HQ_CHECK(s && out, HQ_ERR_ARG, "sharemem: null argument");
+    *out = nullptr;
+    const int n = s->n;
+    HQ_CHECK(num_gates >= 0 && (num_gates == 0 || gates), HQ_ERR_ARG, "sharemem: bad gate array");
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    HQ_CHECK((shared_mask & ~all) == 0, HQ_ERR_ARG, "sharemem: shared mask has bits beyond the state");
+    const int k = __builtin_popcountll(shared_mask);
+    HQ_CHECK(k >= 7 && k <= SHM_MAX_TILE, HQ_ERR_ARG, "sharemem: tile must have 7..14 qubits");
+    HQ_CHECK(((size_t)1 << k) * s->amp_bytes <= 160 * 1024, HQ_ERR_ARG, "sharemem: tile exceeds the 160 KiB LDS");
+    const int rb = rb_req > 0 ? rb_req : sharemem_default_rb(s->precision, k);
+    const int tb = k - rb;
+    HQ_CHECK(shape_supported(rb, tb), HQ_ERR_UNSUPPORTED, "sharemem: unsupported register/thread split");
+
+    int tile_of[64], block_of[64];
+    std::vector<int> tile_pos, block_pos;
+    for (int q = 0; q < n; q++) {
+        if ((shared_mask >> q) & 1) { tile_of[q] = (int)tile_pos.size(); block_of[q] = -1; tile_pos.push_back(q); }
+        else { block_of[q] = (int)block_pos.size(); tile_of[q] = -1; block_pos.push_back(q); }
+    }
+
+    std::vector<Lowered> low;
+    low.reserve(num_gates);
+    for (int i = 0; i < num_gates; i++) {
+        const hq_gate& g = gates[i];
+        hq_status st = validate_gate(g, n);
+        if (st != HQ_OK) return st;
+        if (g.type == HQ_ID) continue;
+        Lowered L;
+        L.cls = classify_gate(g);
+        L.needs_reg = !(L.cls == SHM_DIAG || L.cls == SHM_PHASE);
+        L.t_tile = tile_of[g.target];
+        L.t_block = block_of[g.target];
+        if (L.needs_reg && L.t_tile < 0) {
+            set_error("sharemem: gate " + std::to_string(i) + " is not diagonal but its target is outside the tile");
+            return HQ_ERR_ARG;
+        }
+        int cs[2] = {g.control, g.control2};
+        for (int c = 0; c < 2; c++) {
+            L.c_tile[c] = cs[c] >= 0 ? tile_of[cs[c]] : -1;
+            L.c_block[c] = cs[c] >= 0 ? block_of[cs[c]] : -1;
+        }
+        memcpy(L.m, g.mat, sizeof(L.m));
+        low.push_back(L);
+    }
+
+    // greedy stage planning: take gates in order; a stage has at most RB
+    // distinct register targets; a gate that cannot join blocks its qubits
+    struct Stage {
+        std::vector<int> reg_bits, thr_bits;
+        std::vector<int> ops;
+    };
+    std::vector<Stage> stages;
+    std::vector<int> pending(low.size());
+    for (size_t i = 0; i < low.size(); i++) pending[i] = (int)i;
+    do {
+        Stage S;
+        uint32_t regset = 0;
+        int cnt = 0;
+        uint32_t blk_tile = 0;
+        uint64_t blk_block = 0;
+        std::vector<int> rest;
+        for (int idx : pending) {
+            const Lowered& L = low[idx];
+            uint32_t tt = L.t_tile >= 0 ? 1u << L.t_tile : 0;
+            uint64_t tbk = L.t_block >= 0 ? 1ull << L.t_block : 0;
+            for (int c = 0; c < 2; c++) {
+                if (L.c_tile[c] >= 0) tt |= 1u << L.c_tile[c];
+                if (L.c_block[c] >= 0) tbk |= 1ull << L.c_block[c];
+            }
+            bool ok = !(tt & blk_tile) && !(tbk & blk_block);
+            if (ok && L.needs_reg) {
+                uint32_t tb1 = 1u << L.t_tile;
+                if (!(regset & tb1)) {
+                    if (cnt < rb) { regset |= tb1; cnt++; }
+                    else ok = false;
+                }
+            }
+            if (ok) S.ops.push_back(idx);
+            else { blk_tile |= tt; blk_block |= tbk; rest.push_back(idx); }
+        }
+        // fill the register set with the highest remaining tile bits, so the
+        // lanes cover the low (contiguous) tile bits
+        for (int b = k - 1; b >= 0 && cnt < rb; b--)
+            if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
+        for (int b = 0; b < k; b++)
+            if ((regset >> b) & 1) S.reg_bits.push_back(b);
+        // thread bits: lanes 0..2 get three bits with independent swizzle
+        // columns (lowest first), the rest follow in ascending order
+        std::vector<int> free_bits;
+        for (int b = 0; b < k; b++)
+            if (!((regset >> b) & 1)) free_bits.push_back(b);
+        std::vector<int> lane3;
+        std::vector<uint32_t> cols;
+        for (int b : free_bits) {
+            if (lane3.size() == 3) break;
+            cols.push_back(swz_column(b));
+            if (rank3(cols) == (int)cols.size()) lane3.push_back(b);
+            else cols.pop_back();
+        }
+        for (int b : lane3) S.thr_bits.push_back(b);
+        for (int b : free_bits)
+            if (std::find(lane3.begin(), lane3.end(), b) == lane3.end()) S.thr_bits.push_back(b);
+        stages.push_back(std::move(S));
+        pending.swap(rest);
+    } while (!pending.empty());
+
+    // serialize
+    std::vector<ShmOp> ops;
+    std::vector<ShmStage> dst;
+    for (auto& S : stages) {
+        ShmStage d;
+        memset(&d, 0, sizeof(d));
+        int reg_index[SHM_MAX_TILE], thr_index[SHM_MAX_TILE];
+        for (int i = 0; i < SHM_MAX_TILE; i++) reg_index[i] = thr_index[i] = -1;
+        for (int i = 0; i < rb; i++) { d.reg_bits[i] = S.reg_bits[i]; reg_index[S.reg_bits[i]] = i; }
+        for (int i = 0; i < tb; i++) { d.thr_bits[i] = S.thr_bits[i]; thr_index[S.thr_bits[i]] = i; }
+        d.op_begin = (uint32_t)ops.size();
+        for (int idx : S.ops) {
+            const Lowered& L = low[idx];
+            ShmOp o;
+            memset(&o, 0, sizeof(o));
+            o.cls = (uint8_t)L.cls;
+            if (L.t_tile >= 0 && reg_index[L.t_tile] >= 0) { o.where = SHM_IN_REG; o.tpos = (uint8_t)reg_index[L.t_tile]; }
+            else if (L.t_tile >= 0) { o.where = SHM_IN_THREAD; o.tpos = (uint8_t)thr_index[L.t_tile]; }
+            else { o.where = SHM_IN_BLOCK; o.tpos = (uint8_t)L.t_block; }
+            for (int c = 0; c < 2; c++) {
+                if (L.c_tile[c] >= 0) {
+                    if (reg_index[L.c_tile[c]] >= 0) o.creg |= 1u << reg_index[L.c_tile[c]];
+                    else o.cthr |= 1u << thr_index[L.c_tile[c]];
+                } else if (L.c_block[c] >= 0) {
+                    o.cblk |= 1ull << L.c_block[c];
+                }
+            }
+            memcpy(o.m, L.m, sizeof(o.m));
+            ops.push_back(o);
+        }
+        d.op_end = (uint32_t)ops.size();
+        dst.push_back(d);
+    }
+
+    ShmHeader hdr;
+    memset(&hdr, 0, sizeof(hdr));
+    for (size_t j = 0; j < block_pos.size(); j++) hdr.block_phys[j] = 1ull << block_pos[j];
+    for (int i = 0; i < k; i++) { hdr.tile_phys[i] = 1ull << tile_pos[i]; hdr.swz_col[i] = swz_column(i); }
+    hdr.num_block_bits = (uint32_t)block_pos.size();
+    hdr.num_stages = (uint32_t)dst.size();
+    hdr.num_ops = (uint32_t)ops.size();
+
+    size_t bytes = sizeof(ShmHeader) + dst.size() * sizeof(ShmStage) + ops.size() * sizeof(ShmOp);
+    std::vector<char> blob(bytes);
+    memcpy(blob.data(), &hdr, sizeof(hdr));
+    memcpy(blob.data() + sizeof(hdr), dst.data(), dst.size() * sizeof(ShmStage));
+    if (!ops.empty())
+        memcpy(blob.data() + sizeof(hdr) + dst.size() * sizeof(ShmStage), ops.data(), ops.size() * sizeof(ShmOp));
+
+    hq_program p = new hq_program_s();
+    p->kind = 1;
+    p->n = n;
+    p->precision = s->precision;
+    p->tile = k;
+    p->rb = rb;
+    p->tb = tb;
+    p->num_stages = (int)dst.size();
+    p->num_ops = (int)ops.size();
+    p->shared_mask = shared_mask;
+    p->prog_bytes = bytes;
+    hipError_t e = hipMalloc(&p->d_prog, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("sharemem: hipMalloc: ") + hipGetErrorString(e));
+        delete p;
+        return HQ_ERR_OOM;
+    }
+    e = hipMemcpy(p->d_prog, blob.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("sharemem: hipMemcpy: ") + hipGetErrorString(e));
+        (void)hipFree(p->d_prog);
+        delete p;
+        return HQ_ERR_HIP;
+    }
+    *out = p;
+    return HQ_OK;
+}
+
+}  // namespace hq

@@ -1,0 +1,352 @@
+// transmm.hip -- TransMM: bit-permutation transpose + complex GEMM on MFMA.
+//
+// Reference behaviour (src/executor.cpp:533-551, src/schedule.cpp:754-783):
+// cuttExecute moves the group's qubits to the low bit positions (state ->
+// buffer), then cublasZgemm multiplies every run of K = 2^matQubit
+// consecutive amplitudes by the fused K x K matrix (buffer -> state).
+//
+// MI355X design: the transpose is a tiled bit permutation through LDS (tile
+// spans the low bits of both the input and the output so reads and writes
+// are 256-byte runs).  The GEMM is written for MFMA: one wave per 16-row
+// block of U, U's fragments held in registers for the whole (persistent)
+// kernel, B tiles staged through LDS as re/im planes, the complex product as
+// four real MFMAs (v_mfma_f64_16x16x4f64 for complex128,
+// v_mfma_f32_16x16x4f32 with FP32 accumulation for complex64), and the C
+// tile staged back through LDS for coalesced stores.  Both passes ping-pong
+// between the state and the work buffer; the handle's data pointer follows.
+
+#include "hq_ops.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace hq {
+
+// ---- transpose ---------------------------------------------------------------------
+
+constexpr int TR_TILE_BITS = 10;    // 1024 amplitudes per block
+constexpr int TR_THREADS = 256;
+
+struct TransposeArgs {
+    uint64_t out_block_phys[64];   // output bit (mask) of block-index bit j
+    uint64_t in_block_phys[64];    // input bit (mask) of block-index bit j
+    uint64_t in_tile_phys[TR_TILE_BITS];   // input bit of input-tile bit i
+    uint64_t out_tile_phys[TR_TILE_BITS];  // output bit of output-tile bit i
+    uint32_t in2out[TR_TILE_BITS];         // output-tile bit of input-tile bit i
+    uint32_t num_block_bits;
+};
+
+__device__ __forceinline__ uint32_t tr_swz(uint32_t x) { return x ^ ((x >> 5) & 31); }
+
+template <typename R>
+__global__ __launch_bounds__(TR_THREADS) void k_transpose(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                                                          const TransposeArgs* __restrict__ A)
+{
+    __shared__ cplx<R> lds[1 << TR_TILE_BITS];
+    const uint64_t blk = blockIdx.x;
+    uint64_t bin = 0, bout = 0;
+    const uint32_t nbb = A->num_block_bits;
+    for (uint32_t j = 0; j < nbb; j++)
+        if ((blk >> j) & 1) { bin |= A->in_block_phys[j]; bout |= A->out_block_phys[j]; }
+    const uint32_t tid = threadIdx.x;
+    constexpr int TBITS = 8;   // log2(TR_THREADS)
+    constexpr int PER = (1 << TR_TILE_BITS) / TR_THREADS;
+    // per-lane parts of the three index maps
+    uint64_t lane_in = 0, lane_out = 0;
+    uint32_t lane_lds = 0;
+#pragma unroll
+    for (int i = 0; i < TBITS; i++) {
+        if ((tid >> i) & 1) {
+            lane_in |= A->in_tile_phys[i];
+            lane_lds |= 1u << A->in2out[i];
+            lane_out |= A->out_tile_phys[i];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        uint64_t ui = 0;
+        uint32_t ul = 0;
+#pragma unroll
+        for (int i = 0; i < TR_TILE_BITS - TBITS; i++)
+            if ((r >> i) & 1) { ui |= A->in_tile_phys[TBITS + i]; ul |= 1u << A->in2out[TBITS + i]; }
+        lds[tr_swz(lane_lds | ul)] = in[bin | lane_in | ui];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        uint64_t uo = 0;
+#pragma unroll
+        for (int i = 0; i < TR_TILE_BITS - TBITS; i++)
+            if ((r >> i) & 1) uo |= A->out_tile_phys[TBITS + i];
+        uint32_t e = tid | ((uint32_t)r << TBITS);
+        out[bout | lane_out | uo] = lds[tr_swz(e)];
+    }
+}
+
+// Small states (n < 10): one thread per amplitude, direct gather.
+template <typename R>
+__global__ void k_transpose_small(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int n,
+                                  const int* __restrict__ perm_dev)
+{
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ((uint64_t)1 << n)) return;
+    uint64_t src = 0;
+    for (int b = 0; b < n; b++)
+        if ((j >> b) & 1) src |= (uint64_t)1 << perm_dev[b];
+    out[j] = in[src];
+}
+
+struct TransposePlan {
+    TransposeArgs args;
+    int n;
+};
+
+static void plan_transpose(int n, const int* perm, int run_bits, TransposePlan* P)
+{
+    // output bit b takes input bit perm[b]
+    memset(P, 0, sizeof(*P));
+    P->n = n;
+    int inv[64];
+    for (int b = 0; b < n; b++) inv[perm[b]] = b;
+    // tile bits over OUTPUT positions: output low bits, the output positions of
+    // the input low bits, then more output low bits up to TR_TILE_BITS
+    uint64_t T = 0;
+    int a = std::min(run_bits, n);
+    for (int b = 0; b < a; b++) T |= 1ull << b;
+    for (int p = 0; p < a; p++) T |= 1ull << inv[p];
+    for (int b = 0; b < n && __builtin_popcountll(T) < TR_TILE_BITS; b++) T |= 1ull << b;
+    // output-tile bits in ascending output position
+    int out_bits[TR_TILE_BITS], no = 0;
+    for (int b = 0; b < n; b++) if ((T >> b) & 1) out_bits[no++] = b;
+    // input-tile bits: the same qubits in ascending INPUT position
+    int in_pos[TR_TILE_BITS], ni = 0;
+    for (int p = 0; p < n; p++) if ((T >> inv[p]) & 1) in_pos[ni++] = p;
+    for (int i = 0; i < TR_TILE_BITS; i++) {
+        P->args.in_tile_phys[i] = 1ull << in_pos[i];
+        P->args.out_tile_phys[i] = 1ull << out_bits[i];
+        int ob = inv[in_pos[i]];
+        int rank = 0;
+        for (int k = 0; k < TR_TILE_BITS; k++) if (out_bits[k] == ob) rank = k;
+        P->args.in2out[i] = (uint32_t)rank;
+    }
+    int j = 0;
+    for (int b = 0; b < n; b++) {
+        if ((T >> b) & 1) continue;
+        P->args.out_block_phys[j] = 1ull << b;
+        P->args.in_block_phys[j] = 1ull << perm[b];
+        j++;
+    }
+    P->args.num_block_bits = (uint32_t)j;
+}
+
+hq_status transpose_prepare(hq_state s, const int* perm, void** dev_args)
+{
+    // device-resident launch arguments, built once per program
+    *dev_args = nullptr;
+    const int n = s->n;
+    if (n < TR_TILE_BITS) {
+        HQ_HIP(hipMalloc(dev_args, sizeof(int) * n));
+        HQ_HIP(hipMemcpy(*dev_args, perm, sizeof(int) * n, hipMemcpyHostToDevice));
+        return HQ_OK;
+    }
+    TransposePlan P;
+    plan_transpose(n, perm, s->precision == HQ_C128 ? 4 : 5, &P);
+    HQ_HIP(hipMalloc(dev_args, sizeof(TransposeArgs)));
+    HQ_HIP(hipMemcpy(*dev_args, &P.args, sizeof(TransposeArgs), hipMemcpyHostToDevice));
+    return HQ_OK;
+}
+
+hq_status transpose_launch(hq_state s, const void* dev_args, void* in, void* out)
+{
+    const int n = s->n;
+    if (n < TR_TILE_BITS) {
+        unsigned g = (unsigned)((s->amps + 255) / 256);
+        if (s->precision == HQ_C128)
+            hipLaunchKernelGGL(k_transpose_small<double>, dim3(g), dim3(256), 0, s->stream,
+                               (const cplx<double>*)in, (cplx<double>*)out, n, (const int*)dev_args);
+        else
+            hipLaunchKernelGGL(k_transpose_small<float>, dim3(g), dim3(256), 0, s->stream,
+                               (const cplx<float>*)in, (cplx<float>*)out, n, (const int*)dev_args);
+        HQ_HIP(hipGetLastError());
+        return HQ_OK;
+    }
+    uint64_t blocks = s->amps >> TR_TILE_BITS;
+    const TransposeArgs* A = (const TransposeArgs*)dev_args;
+    if (s->precision == HQ_C128)
+        hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)blocks), dim3(TR_THREADS), 0, s->stream,
+                           (const cplx<double>*)in, (cplx<double>*)out, A);
+    else
+        hipLaunchKernelGGL(k_transpose<float>, dim3((unsigned)blocks), dim3(TR_THREADS), 0, s->stream,
+                           (const cplx<float>*)in, (cplx<float>*)out, A);
+    HQ_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+// ---- dense K x K on runs of K amplitudes -------------------------------------------------
+
+// Generic path (any K <= 128): one thread per output amplitude, U from global.
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_simple(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                                                      const R* __restrict__ ure, const R* __restrict__ uim,
+                                                      int mq, uint64_t n_amps)
+{
+    const uint64_t K = (uint64_t)1 << mq;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_amps; i += stride) {
+        uint64_t col = i >> mq, row = i & (K - 1);
+        const cplx<R>* b = in + col * K;
+        double sr = 0, si = 0;
+        for (uint64_t j = 0; j < K; j++) {
+            double ar = ure[j * K + row], ai = uim[j * K + row];
+            cplx<R> v = b[j];
+            sr += ar * (double)v.x - ai * (double)v.y;
+            si += ar * (double)v.y + ai * (double)v.x;
+        }
+        cplx<R> o;
+        o.x = (R)sr;
+        o.y = (R)si;
+        out[i] = o;
+    }
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename R> struct MfmaAcc;
+template <> struct MfmaAcc<double> { using T = v4d; };
+template <> struct MfmaAcc<float> { using T = v4f; };
+
+__device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// One wave per 16-row block of U (K/16 waves), NT columns per tile.
+// MFMA 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
+// D[4*(l/16)+v][l%16] for v = 0..3.
+template <typename R, int MQ, int NT>
+__global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx<R>* __restrict__ in,
+                                                                     cplx<R>* __restrict__ out,
+                                                                     const R* __restrict__ ure,
+                                                                     const R* __restrict__ uim, uint64_t ncols)
+{
+    constexpr int K = 1 << MQ;
+    constexpr int KS = K / 4;          // k-steps
+    constexpr int NCB = NT / 16;       // column blocks per tile
+    constexpr int NTP = NT + 1;        // padded plane row
+    using Acc = typename MfmaAcc<R>::T;
+    __shared__ R bre[K * NTP];
+    __shared__ R bim[K * NTP];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nthreads = blockDim.x;
+    const int arow = wave * 16 + (lane & 15);
+    const int kq = lane >> 4;          // 0..3
+
+    // U fragments of this wave's row block: A[row][4q + kq]
+    R ar[KS], ai[KS];
+#pragma unroll
+    for (int q = 0; q < KS; q++) {
+        int kk = 4 * q + kq;
+        ar[q] = ure[kk * K + arow];     // column-major: U(row, col) at col*K + row
+        ai[q] = uim[kk * K + arow];
+    }
+
+    const uint64_t ntiles = ncols / NT;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const cplx<R>* __restrict__ src = in + tile * NT * K;
+        // stage B (K x NT) as planes bre[k][col]
+        for (int e = threadIdx.x; e < NT * K; e += nthreads) {
+            cplx<R> v = src[e];
+            int col = e / K, k = e % K;
+            bre[k * NTP + col] = v.x;
+            bim[k * NTP + col] = v.y;
+        }
+        __syncthreads();
+        Acc cr[NCB], ci[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; c++) {
+            cr[c] = Acc{0, 0, 0, 0};
+            ci[c] = Acc{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int q = 0; q < KS; q++) {
+            const int kk = 4 * q + kq;
+#pragma unroll
+            for (int c = 0; c < NCB; c++) {
+                R br = bre[kk * NTP + c * 16 + (lane & 15)];
+                R bi = bim[kk * NTP + c * 16 + (lane & 15)];
+                cr[c] = mfma16x16x4(ar[q], br, cr[c]);
+                cr[c] = mfma16x16x4(ai[q], -bi, cr[c]);
+                ci[c] = mfma16x16x4(ar[q], bi, ci[c]);
+                ci[c] = mfma16x16x4(ai[q], br, ci[c]);
+            }
+        }
+        __syncthreads();
+        // C tile back into the planes (as [row][col]) for a coalesced store
+#pragma unroll
+        for (int c = 0; c < NCB; c++) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                int row = wave * 16 + 4 * (lane >> 4) + v;
+                int col = c * 16 + (lane & 15);
+                bre[row * NTP + col] = cr[c][v];
+                bim[row * NTP + col] = ci[c][v];
+            }
+        }
+        __syncthreads();
+        cplx<R>* __restrict__ dst = out + tile * NT * K;
+        for (int e = threadIdx.x; e < NT * K; e += nthreads) {
+            int col = e / K, k = e % K;
+            cplx<R> v;
+            v.x = bre[k * NTP + col];
+            v.y = bim[k * NTP + col];
+            dst[e] = v;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R, int MQ, int NT>
+static hq_status launch_mfma(hq_state s, const R* ure, const R* uim, void* in, void* out)
+{
+    constexpr int K = 1 << MQ;
+    uint64_t ncols = s->amps / K;
+    uint64_t ntiles = ncols / NT;
+    uint64_t grid = std::min<uint64_t>(ntiles, 2048);
+    hipLaunchKernelGGL((k_dense_mfma<R, MQ, NT>), dim3((unsigned)grid), dim3(64 * (K / 16)), 0, s->stream,
+                       (const cplx<R>*)in, (cplx<R>*)out, ure, uim, ncols);
+    HQ_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+template <typename R>
+static hq_status launch_dense_t(hq_state s, const void* d_mat, int mq, void* in, void* out)
+{
+    const uint64_t K = (uint64_t)1 << mq;
+    const R* ure = (const R*)d_mat;
+    const R* uim = ure + K * K;
+    // MFMA path needs whole column tiles
+    if (mq == 4 && s->amps >= 16 * 64) return launch_mfma<R, 4, 64>(s, ure, uim, in, out);
+    if (mq == 5 && s->amps >= 32 * 64) return launch_mfma<R, 5, 64>(s, ure, uim, in, out);
+    if (mq == 6 && s->amps >= 64 * 64) return launch_mfma<R, 6, 64>(s, ure, uim, in, out);
+    if (mq == 7 && s->amps >= 128 * 32) return launch_mfma<R, 7, 32>(s, ure, uim, in, out);
+    uint64_t blocks = std::min<uint64_t>((s->amps + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(k_dense_simple<R>, dim3((unsigned)blocks), dim3(256), 0, s->stream, (const cplx<R>*)in,
+                       (cplx<R>*)out, ure, uim, mq, s->amps);
+    HQ_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+hq_status launch_dense(hq_state s, const void* d_mat, int mq, void* in, void* out)
+{
+    return s->precision == HQ_C128 ? launch_dense_t<double>(s, d_mat, mq, in, out)
+                                   : launch_dense_t<float>(s, d_mat, mq, in, out);
+}
+
+}  // namespace hq

@@ -1,0 +1,92 @@
+// circuit.h -- public Circuit API (src/circuit.h:9-46), single MI355X GPU.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "compiler.h"
+#include "executor.h"
+#include "gate.h"
+#include "schedule.h"
+
+qreal zero_wrapper(qreal x);   // src/utils.cpp:77-84
+
+struct ResultItem {
+    ResultItem() = default;
+    ResultItem(const qindex& idx, const qComplex& amp) : idx(idx), amp(amp) {}
+    qindex idx = 0;
+    qComplex amp;
+    std::string str() const;   // "%lld %.12f: %.12f %.12f\n" (src/circuit.h:14-16)
+    void print() const;
+    bool operator<(const ResultItem& b) const { return idx < b.idx; }
+};
+
+namespace hq {
+// Raised by the C++ API where the reference calls exit(1).
+struct Error : std::runtime_error {
+    hq_status status;
+    Error(hq_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+}  // namespace hq
+
+class Circuit {
+public:
+    explicit Circuit(int numQubits);
+    ~Circuit();
+    Circuit(const Circuit&) = delete;
+    Circuit& operator=(const Circuit&) = delete;
+
+    void addGate(const Gate& gate) { gates.push_back(gate); compiled = false; }
+    void compile();
+    int run(bool copy_back = true, bool destroy = true);
+    void dumpGates() const;
+    void printState();
+    ResultItem ampAt(qindex idx);
+    qComplex ampAtGPU(qindex idx);
+    const int numQubits;
+
+    // ---- status-returning forms used by the C ABI ----------------------------
+    hq_status partitionChecked();   // host-only: schedule without device programs
+    hq_status compileChecked();
+    hq_status runChecked(bool copy_back, bool destroy, long long* us);
+    std::string stateText(bool showSchedule) const;
+
+    // compile-time options of the reference, runtime here
+    CompilerParams params;
+    ExecParams exec;
+    bool referenceProfile = false;
+
+    const std::vector<Gate>& getGates() const { return gates; }
+    const Schedule& getSchedule() const { return schedule; }
+    hq_state deviceState() const { return state; }
+    const std::vector<int>& finalPos() const { return schedule.finalState.pos; }
+    double lastTimes[4] = {0, 0, 0, 0};   // ms by launch kind (profile on)
+    std::vector<std::pair<int, double>> lastLaunches;
+    bool profileEvents = false;
+    double compileMs = 0;
+
+    qindex toPhysicalID(qindex idx) const;   // src/circuit.cpp:95-103
+    qindex toLogicID(qindex idx) const;      // src/circuit.cpp:105-113
+
+private:
+    bool needsBuffer() const;
+    hq_status ensureState();
+    void freeState();
+
+    std::vector<Gate> gates;
+    Schedule schedule;
+    bool compiled = false;
+    hq_state state = nullptr;
+    bool stateHasBuffer = false;
+    std::vector<qComplex> result;
+};
+
+// Logger of the reference (src/logger.h:9-48): "Logger: <line>" at exit.
+// Off by default, like the reference's SHOW_SUMMARY build option.
+class Logger {
+public:
+    static void add(const char* format, ...);
+    static void print();
+    static void enable(bool on);
+};

@@ -1,0 +1,101 @@
+// executor.cpp -- lowering of schedule groups to device programs.
+//
+// PerGate (OShareMem) groups: the tile is the group's related qubits topped
+// up to the tile size from the lowest physical positions, gates are mapped to
+// physical positions through the current layout (src/executor.cpp:462-493,
+// 585-632); the device layer then splits the tile into register/thread bits
+// and plans the register stages.  BLAS (TransMM) groups: FP64 fusion on the
+// host (src/schedule.cpp:578-702) and a transpose + dense program.
+
+#include "executor.h"
+
+#include <algorithm>
+
+#include "hq_internal.h"
+
+qindex Executor::fillRelated(qindex related, const State& st, int size)
+{
+    int cnt = __builtin_popcountll((unsigned long long)related);
+    for (int i = 0; i < (int)st.layout.size() && cnt < size; i++) {
+        qindex b = qindex(1) << st.layout[i];
+        if (!(related & b)) { related |= b; cnt++; }
+    }
+    return related;
+}
+
+// Tiles smaller than 7 qubits (states below 7 qubits) run the group's gates
+// through the plain kernels: the register kernel needs 64 lanes x 2 registers.
+static const int kMinTile = 7;
+
+hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& p)
+{
+    State st(n);
+    for (auto& lg : sch.localGroups) {
+        for (auto& gg : lg.fullGroups) {
+            if (gg.program) { hq_program_destroy(gg.program); gg.program = nullptr; }
+            if (gg.backend == Backend::PerGate) {
+                const int k = std::min(p.localSize, n);
+                if (__builtin_popcountll((unsigned long long)gg.relatedQubits) > k) {
+                    hq::set_error("executor: group has more related qubits than the tile");
+                    return HQ_ERR_ARG;
+                }
+                if (k >= kMinTile) {
+                    qindex rel = fillRelated(gg.relatedQubits, st, k);
+                    uint64_t mask = 0;
+                    for (int q = 0; q < n; q++)
+                        if ((rel >> q) & 1) mask |= 1ull << st.pos[q];
+                    std::vector<hq_gate> hg;
+                    hg.reserve(gg.gates.size());
+                    for (const Gate& g : gg.gates) {
+                        hq_gate h = g.toHQ();
+                        h.target = st.pos[g.targetQubit];
+                        h.control = g.controlQubit >= 0 ? st.pos[g.controlQubit] : -1;
+                        h.control2 = g.controlQubit2 >= 0 ? st.pos[g.controlQubit2] : -1;
+                        hg.push_back(h);
+                    }
+                    hq_status rc = hq::build_sharemem_program(s, hg.data(), (int)hg.size(), mask, p.regQubits,
+                                                              &gg.program);
+                    if (rc != HQ_OK) return rc;
+                    hq_program_info(gg.program, nullptr, &gg.numStages, nullptr);
+                } else {
+                    gg.numStages = 0;   // plain launches at run time
+                }
+                gg.state = st;
+            } else if (gg.backend == Backend::BLAS) {
+                gg.initCPUMatrix();
+                const size_t K = (size_t)1 << gg.matQubit;
+                std::vector<double> m(2 * K * K);
+                for (size_t i = 0; i < K * K; i++) { m[2 * i] = gg.matrix[i].real(); m[2 * i + 1] = gg.matrix[i].imag(); }
+                hq_status rc = hq_program_create_blas(s, gg.cuttPerm.data(), m.data(), gg.matQubit, &gg.program);
+                if (rc != HQ_OK) return rc;
+                st = gg.state;
+            }
+        }
+    }
+    return HQ_OK;
+}
+
+hq_status Executor::run(Schedule& sch, hq_state s)
+{
+    for (auto& lg : sch.localGroups) {
+        for (auto& gg : lg.fullGroups) {
+            hq_status rc;
+            if (gg.program) {
+                rc = hq_program_launch(s, gg.program);
+            } else {
+                // tiny state: same group, plain kernels, physical positions
+                std::vector<hq_gate> hg;
+                for (const Gate& g : gg.gates) {
+                    hq_gate h = g.toHQ();
+                    h.target = gg.state.pos[g.targetQubit];
+                    h.control = g.controlQubit >= 0 ? gg.state.pos[g.controlQubit] : -1;
+                    h.control2 = g.controlQubit2 >= 0 ? gg.state.pos[g.controlQubit2] : -1;
+                    hg.push_back(h);
+                }
+                rc = hq_apply_gates_plain(s, hg.data(), (int)hg.size());
+            }
+            if (rc != HQ_OK) return rc;
+        }
+    }
+    return HQ_OK;
+}

@@ -1,0 +1,23 @@
+// executor.h -- turns a Schedule into device programs and runs them
+// (src/executor.cpp:11-57, 462-551, 585-640, single GPU).
+#pragma once
+
+#include <vector>
+
+#include "schedule.h"
+
+struct ExecParams {
+    int localSize = 12;   // OShareMem tile qubits (fillRelatedQubits target)
+    int regQubits = 0;    // register qubits per thread (0: default for the tile)
+};
+
+class Executor {
+public:
+    // Lower every group and upload its program (once per compile).
+    static hq_status prepare(Schedule& sch, int numQubits, hq_state s, const ExecParams& p);
+    // Launch every group in order on s (asynchronous).
+    static hq_status run(Schedule& sch, hq_state s);
+    // fillRelatedQubits (src/executor.cpp:585-596): top the group's qubits up
+    // to `size` with the lowest physical positions.
+    static qindex fillRelated(qindex related, const State& st, int size);
+};

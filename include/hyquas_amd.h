@@ -1,0 +1,297 @@
+/*
+ * hyquas_amd.h -- C ABI of the MI355X state-vector simulator.
+ *
+ * Two layers, both plain C (no C++/HIP/torch types cross this boundary):
+ *
+ *  1. Device layer (hq_state_* / hq_apply_* / hq_program_*): the functions
+ *     the reference's executor reaches through src/kernel.h:11-32 plus its
+ *     cuTT/cuBLAS calls.  Qubit numbers here are PHYSICAL bit positions of
+ *     the amplitude index.
+ *  2. Circuit layer (hq_circuit_*): the public Circuit/Gate API of
+ *     src/circuit.h:20-46 and src/gate.h:30-59 for hosts that are not C++
+ *     (ctypes, cgo, JNI).  Qubit numbers here are LOGICAL.
+ *
+ * Every entry point returns an hq_status; none exits the process (the
+ * reference calls exit(1) on any error, src/utils.h:106-144).  The text of
+ * the most recent error on the calling thread is in hq_last_error().
+ *
+ * Threading: a state or circuit handle must be used by one host thread at
+ * a time; distinct handles are independent (each state owns its stream and
+ * device buffers, so nothing is shared through a global gate table as in
+ * src/kernelOpt.cu:15).
+ */
+#ifndef HYQUAS_AMD_H
+#define HYQUAS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+typedef enum hq_status {
+    HQ_OK = 0,
+    HQ_ERR_ARG = 1,          /* invalid argument (qubit out of range, ...)      */
+    HQ_ERR_NO_DEVICE = 2,    /* no HIP device visible                           */
+    HQ_ERR_HIP = 3,          /* a HIP runtime call failed                       */
+    HQ_ERR_OOM = 4,          /* device or host allocation failed                */
+    HQ_ERR_PARSE = 5,        /* malformed circuit text                          */
+    HQ_ERR_STATE = 6,        /* call out of order (e.g. run before compile)     */
+    HQ_ERR_UNSUPPORTED = 7   /* valid request this build does not implement     */
+} hq_status;
+
+/* ---- gate types: the values of GateType, src/gate.h:7-9 ------------------ */
+enum {
+    HQ_CCX = 0, HQ_CNOT, HQ_CY, HQ_CZ, HQ_CRX, HQ_CRY, HQ_CU1, HQ_CRZ,
+    HQ_U1, HQ_U2, HQ_U3, HQ_H, HQ_X, HQ_Y, HQ_Z, HQ_S, HQ_SDG, HQ_T, HQ_TDG,
+    HQ_RX, HQ_RY, HQ_RZ, HQ_TOTAL, HQ_ID, HQ_GII, HQ_GZZ, HQ_GOC, HQ_GCC
+};
+
+/* ---- amplitude precision ------------------------------------------------- */
+enum { HQ_C128 = 0,   /* complex128 state (qreal = double, USE_DOUBLE)       */
+       HQ_C64 = 1 };  /* complex64 state (qreal = float), FP64 host fusion   */
+
+/* ---- schedule backends (Backend, src/schedule.h:9-11) -------------------- */
+enum { HQ_BACKEND_NONE = 0, HQ_BACKEND_PERGATE = 1, HQ_BACKEND_BLAS = 2 };
+
+/* ---- compile modes (the CMake BACKEND option, CMakeLists.txt:16-42) ------- */
+enum { HQ_MODE_SERIAL = 0,   /* plain per-gate kernels                       */
+       HQ_MODE_GROUP = 1,    /* OShareMem groups only                        */
+       HQ_MODE_BLAS = 3,     /* TransMM groups only                          */
+       HQ_MODE_MIX = 4 };    /* hybrid: cost model picks per group           */
+
+/* A gate.  Mirrors Gate (src/gate.h:11-19) with the matrix in the order of
+ * KernelGate (src/gate.h:75): re00 im00 re01 im01 re10 im10 re11 im11. */
+typedef struct hq_gate {
+    int32_t type;       /* HQ_* gate type                                   */
+    int32_t target;
+    int32_t control;    /* -1 if none                                       */
+    int32_t control2;   /* -1 if none (only CCX has two controls)           */
+    double  mat[8];
+} hq_gate;
+
+/* Opaque handles. */
+typedef struct hq_state_s   *hq_state;
+typedef struct hq_program_s *hq_program;
+typedef struct hq_circuit_s *hq_circuit;
+
+/* ======================================================================== */
+/* Library                                                                  */
+/* ======================================================================== */
+
+/* Text of the last error on this thread ("" if none). */
+const char *hq_last_error(void);
+
+/* Number of visible HIP devices (replaces the device query inside
+ * MyGlobalVars::init, src/utils.cpp:17-24).  HQ_ERR_NO_DEVICE if none. */
+hq_status hq_device_count(int *count);
+
+/* Build a gate from its type and parameters: the Gate factories of
+ * src/gate.cpp:9-342 (Gate::H(t), Gate::RX(t, angle), Gate::CCX(c1, c2, t)...).
+ * params: RX/RY/RZ/U1/CRX/CRY/CRZ/CU1 one angle; U2 (phi, lambda);
+ * U3 (theta, phi, lambda); GOC/GCC (re, im). */
+hq_status hq_gate_make(int type, int control2, int control, int target,
+                       const double *params, int nparams, hq_gate *out);
+
+/* ======================================================================== */
+/* Device layer                                                             */
+/* ======================================================================== */
+
+/* Allocate a 2^num_qubits state on the current device, with an equal-size
+ * work buffer when with_buffer != 0 (needed by hq_apply_transpose /
+ * hq_apply_dense).  Replaces the allocation half of kernelInit
+ * (src/kernelSimple.cu:9-27), which doubles the allocation for blas/mix. */
+hq_status hq_state_create(int num_qubits, int precision, int with_buffer, hq_state *out);
+
+/* Free a state (kernelDestroy, src/kernelSimple.cu:528-529). */
+hq_status hq_state_destroy(hq_state s);
+
+/* Set |0...0> (the zero + amp[0]=1 half of kernelInit,
+ * src/kernelSimple.cu:21-31).  Asynchronous on the state's stream. */
+hq_status hq_state_reset(hq_state s);
+
+/* Copy the whole state to/from host memory; count must be 2^n amplitudes of
+ * the state's precision (interleaved re, im).  Synchronous.  Download
+ * replaces kernelDeviceToHost (src/kernelSimple.cu:524-526). */
+hq_status hq_state_download(hq_state s, void *host, uint64_t count);
+hq_status hq_state_upload(hq_state s, const void *host, uint64_t count);
+
+/* One amplitude, widened to double (kernelGetAmp, src/kernelSimple.cu:518-522). */
+hq_status hq_state_get_amp(hq_state s, uint64_t index, double *re, double *im);
+
+/* Wait for all work queued on the state (Executor::finalize's stream sync,
+ * src/executor.cpp:634-640). */
+hq_status hq_state_sync(hq_state s);
+
+/* Number of qubits / precision of a state. */
+hq_status hq_state_info(hq_state s, int *num_qubits, int *precision);
+
+/* Plain method: one HBM pass per gate, in order (kernelExecSimple,
+ * src/kernelSimple.cu:335-446).  Asynchronous. */
+hq_status hq_apply_gates_plain(hq_state s, const hq_gate *gates, int num_gates);
+
+/* OShareMem group: compile the gates (physical qubits) into a device-resident
+ * program for the tile spanned by shared_mask (physical positions held in
+ * LDS/registers; every non-diagonal target must be in it).  Replaces the
+ * lowering + copyGatesToSymbol of src/executor.cpp:462-493 and
+ * src/kernelOpt.cu:499-501.  The program stays valid until destroyed and may
+ * be launched on any state with the same qubit count and precision. */
+hq_status hq_program_create_sharemem(hq_state s, const hq_gate *gates, int num_gates,
+                                     uint64_t shared_mask, hq_program *out);
+
+/* TransMM group as one program: bit-permutation transpose by perm (output
+ * bit i takes input bit perm[i], as cuttPlan is called at
+ * src/schedule.cpp:767-774) followed by the column-major K x K matrix
+ * (K = 2^mat_qubits, FP64, re/im interleaved, mat[(col*K + row)*2]) applied
+ * to every run of K consecutive amplitudes (cublasZgemm at
+ * src/executor.cpp:544-549).  perm may be NULL for the identity.  Needs a
+ * state created with a buffer. */
+hq_status hq_program_create_blas(hq_state s, const int *perm, const double *mat_colmajor,
+                                 int mat_qubits, hq_program *out);
+
+/* Launch a program on a state (launchExecutor, src/kernelOpt.cu:503-506, or
+ * applyBlasGroup, src/executor.cpp:533-551).  Asynchronous. */
+hq_status hq_program_launch(hq_state s, hq_program p);
+hq_status hq_program_destroy(hq_program p);
+
+/* Describe a compiled OShareMem program: tile qubits, register stages and
+ * lowered op count (for tests and the cost model). */
+hq_status hq_program_info(hq_program p, int *tile_qubits, int *num_stages, int *num_ops);
+
+/* One-shot forms of the above (create + launch + destroy). */
+hq_status hq_apply_sharemem(hq_state s, const hq_gate *gates, int num_gates, uint64_t shared_mask);
+hq_status hq_apply_transpose(hq_state s, const int *perm);                  /* cuttExecute  */
+hq_status hq_apply_dense(hq_state s, const double *mat_colmajor, int mat_qubits); /* cublasZgemm */
+
+/* Probability that qubit `target` (physical) reads 0 (kernelMeasure,
+ * src/kernelSimple.cu:497-516).  FP64 accumulation.  Synchronous. */
+hq_status hq_measure_zero(hq_state s, int target, double *p0);
+
+/* Squared norm sum |a_i|^2, FP64 accumulation (Kahan-compensated per thread
+ * for complex64 states).  Synchronous. */
+hq_status hq_state_norm(hq_state s, double *norm2);
+
+/* <Z_q> for every physical qubit q (z[n]) and <Z_q Z_{q+1}> (zz[n-1]); either
+ * pointer may be NULL.  One HBM pass, FP64 accumulation.  Synchronous. */
+hq_status hq_expect_z(hq_state s, double *z, double *zz);
+
+/* Overlap <a|b> of two states with the same qubit count (any precisions),
+ * FP64 accumulation.  Synchronous. */
+hq_status hq_state_inner(hq_state a, hq_state b, double *re, double *im);
+
+/* Device-side timing of the most recent launches (HIP events on the state's
+ * stream).  enable != 0 starts recording one event pair per program launch
+ * or plain gate batch; hq_profile_read returns up to cap entries of
+ * {kind, ms} (kind: 0 plain, 1 sharemem, 2 transpose, 3 dense) and clears
+ * the log.  Synchronizes the stream. */
+hq_status hq_profile_enable(hq_state s, int enable);
+hq_status hq_profile_read(hq_state s, int *kinds, double *ms, int cap, int *count);
+
+/* ======================================================================== */
+/* Circuit layer (src/circuit.h:20-46)                                      */
+/* ======================================================================== */
+
+hq_status hq_circuit_create(int num_qubits, hq_circuit *out);          /* Circuit(n)   */
+hq_status hq_circuit_destroy(hq_circuit c);
+hq_status hq_circuit_add_gate(hq_circuit c, const hq_gate *g);         /* addGate      */
+hq_status hq_circuit_num_gates(hq_circuit c, int *count);
+hq_status hq_circuit_num_qubits(hq_circuit c, int *num_qubits);
+
+/* Parse the OpenQASM subset of main.cpp:68-231 from a file or a string. */
+hq_status hq_circuit_load_qasm(const char *path, hq_circuit *out);
+hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
+
+/* Options (the compile-time CMake options of the reference, runtime here):
+ *   "mode"        HQ_MODE_* (default HQ_MODE_MIX)
+ *   "precision"   HQ_C128 / HQ_C64 (default HQ_C128)
+ *   "profile"     0 = gfx950 parameters (default), 1 = reference parameters
+ *                 (LOCAL_QUBIT_SIZE 10, COALESCE 3, V100 cost tables)
+ *   "share_qubits" OShareMem tile qubits (default 12; 6..14)
+ *   "coalesce"    COALESCE_GLOBAL (default 3)
+ *   "mat_min", "mat_max"  TransMM group qubit range in mix mode (4, 7)
+ *   "blas_mat"    BLAS_MAT_LIMIT for HQ_MODE_BLAS (default 6)
+ *   "min_mat"     MIN_MAT_SIZE (default 4)
+ *   "reg_qubits"  register qubits per thread in OShareMem (0 = auto)
+ *   "window"      gates considered per group (default 512)
+ *   "profile_events" 1 = time every launch with HIP events
+ * Must be set before compile. */
+hq_status hq_circuit_set_option(hq_circuit c, const char *key, int64_t value);
+
+/* Load an MI355X cost-model parameter file (the format of
+ * evaluator-preprocess/parameter-files, extended; see hq_calibrate). */
+hq_status hq_circuit_load_cost_model(hq_circuit c, const char *path);
+
+/* Partition + lower + fuse (Circuit::compile, src/circuit.cpp:177-210).
+ * Uploads every group program to the device. */
+hq_status hq_circuit_compile(hq_circuit c);
+
+/* The partition half of compile only (Compiler::run, src/compiler.cpp:70-165):
+ * host work, no device needed; hq_circuit_schedule* then describe it. */
+hq_status hq_circuit_partition(hq_circuit c);
+
+/* Circuit::run (src/circuit.cpp:16-71): allocate (first call) and reset the
+ * state, run every group, wait.  *us receives the wall time of the group
+ * loop in microseconds (the reference's "Time Cost").  copy_back keeps a
+ * host copy for hq_circuit_amp_at / hq_circuit_print_state; destroy frees
+ * the device state afterwards (keep it for hq_circuit_amp_at_gpu and the
+ * device readouts). */
+hq_status hq_circuit_run(hq_circuit c, int copy_back, int destroy, int64_t *us);
+
+/* ampAt (host copy, needs copy_back) / ampAtGPU (device, needs destroy=0);
+ * logical index (src/circuit.cpp:115-151). */
+hq_status hq_circuit_amp_at(hq_circuit c, uint64_t index, double *re, double *im);
+hq_status hq_circuit_amp_at_gpu(hq_circuit c, uint64_t index, double *re, double *im);
+
+/* printState text (src/circuit.cpp:250-309) into buf (NUL-terminated when
+ * it fits); *len receives the full length.  show_schedule adds the 1<<i and
+ * 2^n-1 lines of the SHOW_SCHEDULE build. */
+hq_status hq_circuit_print_state(hq_circuit c, int show_schedule, char *buf,
+                                 size_t cap, size_t *len);
+
+/* Borrow the device state after run(destroy=0) for the device readouts
+ * (physical indices: combine with hq_circuit_final_pos). */
+hq_status hq_circuit_state(hq_circuit c, hq_state *out);
+
+/* Final logical->physical map (schedule.finalState.pos), pos[n]. */
+hq_status hq_circuit_final_pos(hq_circuit c, int *pos);
+
+/* Schedule summary after compile: number of groups, and per group its
+ * backend, gate count, related (logical) qubit mask, TransMM matrix qubits,
+ * OShareMem register stages; any array may be NULL. */
+hq_status hq_circuit_schedule(hq_circuit c, int *num_groups, int cap, int *backend,
+                              int *num_gates, uint64_t *related, int *mat_qubit,
+                              int *num_stages);
+
+/* Gate order of the schedule: order[i] = index (in addGate order) of the
+ * i-th gate executed; cap >= number of gates. */
+hq_status hq_circuit_schedule_order(hq_circuit c, int *order, int cap);
+
+/* Timing of the last run with option "profile_events" = 1: total device ms
+ * per launch kind (kinds as in hq_profile_read). */
+hq_status hq_circuit_last_times(hq_circuit c, double *plain_ms, double *sharemem_ms,
+                                double *transpose_ms, double *dense_ms);
+
+/* Per-launch device times of the last run with option "profile_events" = 1:
+ * kinds[i] as in hq_profile_read, ms[i] its duration; *count = launches. */
+hq_status hq_circuit_launch_times(hq_circuit c, int *kinds, double *ms, int cap, int *count);
+
+/* The circuit's gates in addGate order (logical qubits). */
+hq_status hq_circuit_get_gates(hq_circuit c, hq_gate *out, int cap, int *count);
+
+/* Generators of the synthetic workloads (portable: xoshiro256** seeded by
+ * SplitMix64, doubles as (x >> 11) * 2^-53).
+ *  random layers: each layer puts one of {H, RX, RY, RZ, T} (uniform) on
+ *    every qubit, theta ~ U[0, 2pi), then CNOTs on a uniformly random perfect
+ *    matching (Fisher-Yates; pairs (p[2i], p[2i+1]) as (control, target)).
+ *  hardware-efficient ansatz: per layer RY(theta), RZ(phi) on every qubit,
+ *    then CZ(i, i+1) for i < n-1.
+ * Append to c. */
+hq_status hq_circuit_add_random_layers(hq_circuit c, int layers, uint64_t seed);
+hq_status hq_circuit_add_hea_layers(hq_circuit c, int layers, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYQUAS_AMD_H */

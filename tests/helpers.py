@@ -1,0 +1,133 @@
+"""Test-side helpers: an independent restatement of the synthetic workload
+generators (to check the library's) and circuit builders for the oracle."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import oracle
+
+MASK64 = (1 << 64) - 1
+
+# gate type codes (src/gate.h:7-9)
+CCX, CNOT, CY, CZ, CRX, CRY, CU1, CRZ, U1, U2, U3, H, X, Y, Z, S, SDG, T, TDG, RX, RY, RZ = range(22)
+ID = 23
+
+
+class Rng:
+    """xoshiro256** seeded by SplitMix64 (same definition as hq::Rng)."""
+
+    def __init__(self, seed: int):
+        x = seed & MASK64
+        self.s = []
+        for _ in range(4):
+            x = (x + 0x9E3779B97F4A7C15) & MASK64
+            z = x
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+            self.s.append(z ^ (z >> 31))
+
+    @staticmethod
+    def _rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & MASK64
+
+    def next(self):
+        s = self.s
+        result = (self._rotl((s[1] * 5) & MASK64, 7) * 9) & MASK64
+        t = (s[1] << 17) & MASK64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self._rotl(s[3], 45)
+        return result
+
+    def uniform(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+    def below(self, m):
+        return ((self.next() >> 11) * m) >> 53
+
+
+def random_layers(n, layers, seed):
+    """(type, target, control, params) tuples of the random-layer workload."""
+    rng = Rng(seed)
+    two_pi = 2 * math.acos(-1.0)
+    out = []
+    for _ in range(layers):
+        for q in range(n):
+            k = rng.below(5)
+            if k == 0:
+                out.append((H, q, -1, ()))
+            elif k == 1:
+                out.append((RX, q, -1, (two_pi * rng.uniform(),)))
+            elif k == 2:
+                out.append((RY, q, -1, (two_pi * rng.uniform(),)))
+            elif k == 3:
+                out.append((RZ, q, -1, (two_pi * rng.uniform(),)))
+            else:
+                out.append((T, q, -1, ()))
+        perm = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = rng.below(i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        for i in range(0, n - 1, 2):
+            out.append((CNOT, perm[i + 1], perm[i], ()))
+    return out
+
+
+def hea_layers(n, layers, seed):
+    rng = Rng(seed)
+    two_pi = 2 * math.acos(-1.0)
+    out = []
+    for _ in range(layers):
+        for q in range(n):
+            out.append((RY, q, -1, (two_pi * rng.uniform(),)))
+            out.append((RZ, q, -1, (two_pi * rng.uniform(),)))
+        for q in range(n - 1):
+            out.append((CZ, q + 1, q, ()))
+    return out
+
+
+def to_oracle(spec):
+    """Tuples -> oracle gate array."""
+    arr = np.zeros(len(spec), dtype=oracle.GATE_DTYPE)
+    for i, s in enumerate(spec):
+        t, tgt, c = s[0], s[1], s[2]
+        params = s[3] if len(s) > 3 else ()
+        c2 = s[4] if len(s) > 4 else -1
+        arr[i] = oracle.gate(t, tgt, c, c2, params)
+    return arr
+
+
+ONE_Q = [(H, 0), (X, 0), (Y, 0), (Z, 0), (S, 0), (SDG, 0), (T, 0), (TDG, 0), (RX, 1), (RY, 1), (RZ, 1), (U1, 1),
+         (U2, 2), (U3, 3)]
+TWO_Q = [(CNOT, 0), (CY, 0), (CZ, 0), (CRX, 1), (CRY, 1), (CU1, 1), (CRZ, 1)]
+
+
+def random_mixed(n, m, seed, ccx=True):
+    """Random circuit over every gate type (tuples)."""
+    rs = np.random.default_rng(seed)
+    out = []
+    for _ in range(m):
+        r = rs.random()
+        if r < 0.55:
+            t, npar = ONE_Q[rs.integers(len(ONE_Q))]
+            q = int(rs.integers(n))
+            out.append((t, q, -1, tuple(rs.uniform(0, 2 * np.pi, npar))))
+        elif r < 0.93 or not ccx or n < 3:
+            t, npar = TWO_Q[rs.integers(len(TWO_Q))]
+            a, b = rs.choice(n, 2, replace=False)
+            out.append((t, int(b), int(a), tuple(rs.uniform(0, 2 * np.pi, npar))))
+        else:
+            a, b, c = rs.choice(n, 3, replace=False)
+            out.append((CCX, int(c), int(a), (), int(b)))
+    return out
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))

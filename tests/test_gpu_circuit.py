@@ -1,0 +1,110 @@
+"""GPU parity of the full Circuit path (partition -> lower -> run) against the
+FP64 oracle: amplitudes within 1e-10 relative (2-norm) for complex128,
+1 - fidelity <= 1e-5 for complex64."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import random_layers, random_mixed, rel_err, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _circuit(hq, n, spec):
+    c = hq.Circuit(n)
+    for s in spec:
+        t, tgt, ctl = s[0], s[1], s[2]
+        params = s[3] if len(s) > 3 else ()
+        c2 = s[4] if len(s) > 4 else -1
+        c.addGate(hq.Gate.make(t, c2, ctl, tgt, params))
+    return c
+
+
+def _logical_state(c, n):
+    return np.array([c.ampAt(i) for i in range(1 << n)])
+
+
+@pytest.mark.parametrize("mode", ["serial", "group", "blas", "mix"])
+@pytest.mark.parametrize("n", [5, 9, 14])
+def test_modes_match_oracle(gpu, n, mode):
+    spec = random_mixed(n, 160, seed=n * 7)
+    c = _circuit(gpu, n, spec)
+    c.set_option("mode", dict(serial=0, group=1, blas=3, mix=4)[mode])
+    if mode == "blas":
+        c.set_option("blas_mat", min(6, n))
+    c.compile()
+    c.run()
+    got = _logical_state(c, n)
+    want = oracle.run_plain(to_oracle(spec), n)
+    assert rel_err(got, want) <= 1e-10
+
+
+def test_random_layers_20q_full_compare(gpu):
+    """20 qubits, 20 random layers, seed 0: all 2^20 amplitudes (hybrid mode)."""
+    n = 20
+    c = gpu.Circuit(n).add_random_layers(20, 0)
+    spec = random_layers(n, 20, 0)
+    assert len(c.gates()) == len(spec) == 600
+    c.compile()
+    c.run(copy_back=True, destroy=False)
+    got = c.logical_state()
+    want = oracle.run_plain(to_oracle(spec), n)
+    assert rel_err(got, want) <= 1e-10
+    assert float(np.max(np.abs(got - want))) <= 1e-12
+    assert sum(g["num_gates"] for g in c.schedule()) == 600
+    # ampAt agrees with the device readout
+    for i in (0, 1, 12345, (1 << n) - 1):
+        assert c.ampAt(i) == got[i]
+
+
+def test_random_layers_serial_plain_20q(gpu):
+    n = 20
+    c = gpu.Circuit(n).add_random_layers(20, 0)
+    c.set_option("mode", gpu.HQ_MODE_SERIAL)
+    c.compile()
+    c.run(copy_back=False, destroy=False)
+    want = oracle.run_plain(to_oracle(random_layers(n, 20, 0)), n)
+    assert rel_err(c.logical_state(), want) <= 1e-12
+
+
+def test_random_layers_full_vector(gpu):
+    n = 18
+    c = gpu.Circuit(n).add_random_layers(12, 3)
+    c.compile()
+    c.run(copy_back=False, destroy=False)
+    want = oracle.run_plain(to_oracle(random_layers(n, 12, 3)), n)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+
+
+def test_print_state_matches_oracle_format(gpu):
+    n = 10
+    spec = random_mixed(n, 80, seed=3)
+    c = _circuit(gpu, n, spec)
+    c.set_option("mode", 1)
+    c.compile()
+    c.run()
+    want_psi = oracle.run_plain(to_oracle(spec), n)
+    text = c.stateText()
+    # same lines as the oracle's printState of the oracle state, within print precision
+    ref = oracle.print_state(want_psi, n, list(range(n)))
+    a, b = text.splitlines(), ref.splitlines()
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        ix, rx = x.split(" ", 1)
+        iy, ry = y.split(" ", 1)
+        assert ix == iy
+        vx = [float(v.rstrip(":")) for v in rx.split()]
+        vy = [float(v.rstrip(":")) for v in ry.split()]
+        assert np.allclose(vx, vy, atol=2e-12)
+
+
+def test_c64_fidelity(gpu):
+    n = 16
+    c = gpu.Circuit(n).add_random_layers(10, 11)
+    c.set_option("precision", gpu.HQ_C64)
+    c.compile()
+    c.run(copy_back=False, destroy=False)
+    got = c.logical_state(gpu.HQ_C64)
+    want = oracle.run_plain(to_oracle(random_layers(n, 10, 11)), n)
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+    assert 1 - fid <= 1e-5
