@@ -1,0 +1,177 @@
+"""Benchmark: 30-qubit random-layer circuit, complex128, hybrid partitioner,
+one MI355X.
+
+A step is one full simulation of the circuit from |0...0> on the GPU:
+state reset + every group kernel (OShareMem / TransMM as the partitioner
+chose) + stream sync.  The compiled schedule and every group program are
+resident on the device before timing starts.  Prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30]
+                    [--layers 30] [--seed 0] [--cpu-gates 45] [--no-cpu]
+
+roofline: the dominant kernel is the OShareMem group kernel; its algorithmic
+traffic per launch is one read + one write of the state, 2 * 2^n * 16 bytes,
+divided by its average launch duration (HIP events recorded on the state's
+stream around every launch in the timed steps).
+
+cpu_baseline: the FP64 OpenMP per-gate CPU path (oracle/, kernelSimple.cu
+semantics) timed on the first --cpu-gates gates of the same 30-qubit
+circuit on this host, scaled to the whole circuit.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK = 78.6e12
+
+
+def _traffic_from_profiles(n):
+    """HBM bytes per OShareMem launch measured by rocprofv3 PMC counters, if a
+    summary for this workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"sharemem_traffic_{n}q.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(n, layers, seed, gates_sample):
+    import numpy as np
+
+    import oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import random_layers, to_oracle
+
+    spec = random_layers(n, layers, seed)
+    total = len(spec)
+    sample = to_oracle(spec[:gates_sample])
+    threads = oracle.max_threads()
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    t0 = time.perf_counter()
+    oracle.run_plain(sample, n, psi, threads)
+    dt = time.perf_counter() - t0
+    per_gate = dt / len(sample)
+    return {
+        "value": per_gate * total,
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {len(sample)} of {total} gates of the {n}-qubit circuit ({dt:.1f} s), "
+                  f"per-gate time scaled to the whole circuit",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--layers", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--cpu-gates", type=int, default=45)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the state-vector path shards by global qubits, which is not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import hyquas_amd as hq
+
+    n, layers = a.qubits, a.layers
+    prec = hq.HQ_C128 if a.precision == "c128" else hq.HQ_C64
+    amp_bytes = 16 if prec == hq.HQ_C128 else 8
+    c = hq.Circuit(n).add_random_layers(layers, a.seed)
+    c.set_option("precision", prec)
+    c.set_option("profile_events", 1)
+    t0 = time.perf_counter()
+    c.compile()
+    compile_s = time.perf_counter() - t0
+    sched = c.schedule()
+    ngates = sum(g["num_gates"] for g in sched)
+
+    for _ in range(a.warmup):
+        c.run(copy_back=False, destroy=False)
+
+    launches = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        c.run(copy_back=False, destroy=False)
+        launches.extend(c.launch_times())
+    elapsed = time.perf_counter() - t0
+    sec_per_step = elapsed / a.steps
+
+    # sanity: the state stays normalised
+    st = c.state_view(prec)
+    norm_err = abs(st.norm2() - 1.0)
+
+    shm = [ms for k, ms in launches if k == hq.KIND_SHAREMEM]
+    bytes_per_pass = 2.0 * (1 << n) * amp_bytes
+    roof = None
+    if shm:
+        avg_ms = sum(shm) / len(shm)
+        achieved = bytes_per_pass / (avg_ms * 1e-3) / 1e9
+        traffic = _traffic_from_profiles(n)
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_sharemem", "avg_launch_ms": round(avg_ms, 4),
+                "launches_per_step": len(shm) // a.steps}
+    kinds = {}
+    for k, ms in launches:
+        kinds[k] = kinds.get(k, 0.0) + ms
+    groups_pg = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE)
+    groups_blas = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_BLAS)
+    # whole-step roofline: every group needs one HBM pass (TransMM two)
+    t_ideal = (groups_pg + 2 * groups_blas) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(n, layers, a.seed, a.cpu_gates)
+        except Exception as e:  # reported, not fatal
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": "30q fp64 sim time",
+        "value": round(sec_per_step, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(sec_per_step * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == hq.HQ_C128 else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"random-layer circuit, {n} qubits, {layers} layers (H/RX/RY/RZ/T + CNOT "
+                               f"matching), seed {a.seed}, {a.precision} state, hybrid partitioner",
+                   "qubits": n, "layers": layers, "gates": ngates, "groups_sharemem": groups_pg,
+                   "groups_transmm": groups_blas, "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_hbm_roofline_frac": round(t_ideal / sec_per_step, 4),
+        "device_ms_per_step": {str(k): round(v / a.steps, 3) for k, v in sorted(kinds.items())},
+        "compile_s": round(compile_s, 3),
+        "norm_error": norm_err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

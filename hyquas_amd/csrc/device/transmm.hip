@@ -225,9 +225,14 @@ __device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c)
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Output row of accumulator register v in lane l (column l%16).  Measured on
+// gfx950: v_mfma_f32_16x16x4f32 holds rows 4*(l/16)+v, v_mfma_f64_16x16x4f64
+// holds rows l/16 + 4*v.  Inputs: A[l%16][l/16], B[l/16][l%16] for both.
+template <typename R> __device__ __forceinline__ int mfma_row(int lane, int v);
+template <> __device__ __forceinline__ int mfma_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
+template <> __device__ __forceinline__ int mfma_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
+
 // One wave per 16-row block of U (K/16 waves), NT columns per tile.
-// MFMA 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
-// D[4*(l/16)+v][l%16] for v = 0..3.
 template <typename R, int MQ, int NT>
 __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx<R>* __restrict__ in,
                                                                      cplx<R>* __restrict__ out,
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
         for (int c = 0; c < NCB; c++) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                int row = wave * 16 + 4 * (lane >> 4) + v;
+                int row = wave * 16 + mfma_row<R>(lane, v);
                 int col = c * 16 + (lane & 15);
                 bre[row * NTP + col] = cr[c][v];
                 bim[row * NTP + col] = ci[c][v];
