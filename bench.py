@@ -82,6 +82,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
     ap.add_argument("--cpu-gates", type=int, default=45)
+    ap.add_argument("--share-qubits", type=int, default=0, help="OShareMem tile qubits (0: default)")
+    ap.add_argument("--reg-qubits", type=int, default=0, help="register qubits per thread (0: default)")
+    ap.add_argument("--coalesce", type=int, default=0, help="low qubits always in the tile (0: default)")
+    ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
 
@@ -99,6 +103,13 @@ def main():
     c = hq.Circuit(n).add_random_layers(layers, a.seed)
     c.set_option("precision", prec)
     c.set_option("profile_events", 1)
+    c.set_option("mode", dict(serial=0, group=1, blas=3, mix=4)[a.mode])
+    if a.share_qubits:
+        c.set_option("share_qubits", a.share_qubits)
+    if a.reg_qubits:
+        c.set_option("reg_qubits", a.reg_qubits)
+    if a.coalesce:
+        c.set_option("coalesce", a.coalesce)
     t0 = time.perf_counter()
     c.compile()
     compile_s = time.perf_counter() - t0
@@ -162,7 +173,8 @@ def main():
         "config": {"workload": f"random-layer circuit, {n} qubits, {layers} layers (H/RX/RY/RZ/T + CNOT "
                                f"matching), seed {a.seed}, {a.precision} state, hybrid partitioner",
                    "qubits": n, "layers": layers, "gates": ngates, "groups_sharemem": groups_pg,
-                   "groups_transmm": groups_blas, "parallelism": "single GPU"},
+                   "groups_transmm": groups_blas, "share_qubits": a.share_qubits or None,
+                   "reg_qubits": a.reg_qubits or None, "mode": a.mode, "parallelism": "single GPU"},
         "roofline": roof,
         "step_hbm_roofline_frac": round(t_ideal / sec_per_step, 4),
         "device_ms_per_step": {str(k): round(v / a.steps, 3) for k, v in sorted(kinds.items())},

@@ -81,6 +81,8 @@ def _declare(L):
         "hq_program_launch": (S, [c_void_p, c_void_p]),
         "hq_program_destroy": (S, [c_void_p]),
         "hq_program_info": (S, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "hq_program_specialize": (S, [c_void_p]),
+        "hq_program_is_specialized": (S, [c_void_p, POINTER(c_int)]),
         "hq_apply_sharemem": (S, [c_void_p, POINTER(HQGate), c_int, c_uint64]),
         "hq_apply_transpose": (S, [c_void_p, POINTER(c_int)]),
         "hq_apply_dense": (S, [c_void_p, POINTER(c_double), c_int]),
@@ -261,6 +263,16 @@ class Program:
         t, s, o = c_int(), c_int(), c_int()
         _check(lib().hq_program_info(self.handle, byref(t), byref(s), byref(o)))
         return t.value, s.value, o.value
+
+    def specialize(self):
+        """Compile a kernel specialised to this group's structure (hiprtc)."""
+        _check(lib().hq_program_specialize(self.handle))
+        return self
+
+    def is_specialized(self) -> bool:
+        y = c_int()
+        _check(lib().hq_program_is_specialized(self.handle, byref(y)))
+        return bool(y.value)
 
     def close(self):
         if self.handle:

@@ -80,7 +80,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     objs = [_obj(s) for s in dev + host]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-lgomp", "-Wl,-soname,libhyquas_amd.so"], verbose)
+         ["-lgomp", "-lhiprtc", "-Wl,-soname,libhyquas_amd.so"], verbose)
     os.replace(tmp, LIB)
     _run([HIPCC, "--offload-arch=" + ARCH, "-o", CLI, _obj(main_src), "-L" + HERE, "-lhyquas_amd",
           "-Wl,-rpath,$ORIGIN", "-lgomp"], verbose)

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "hyquas_amd.h"
+#include "hq_internal.h"
 
 namespace hq {
 
@@ -86,11 +87,15 @@ struct ShmOp {
     uint8_t pad0;
     uint32_t creg;      // control mask over register-index bits
     uint32_t cthr;      // control mask over thread-index bits
-    uint32_t pad1;
+    uint32_t flags;     // SHM_FLAG_*
     uint64_t cblk;      // control mask over block-index bits
     double m[8];        // coefficients, re/im interleaved, row-major 2x2
 };
 static_assert(sizeof(ShmOp) == 88, "ShmOp layout");
+
+// the op's coefficients are fixed by its gate type (H, T, CNOT, ...), so a
+// specialised kernel may embed them as constants
+constexpr uint32_t SHM_FLAG_CONST = 1;
 
 constexpr int SHM_MAX_TILE = 14;
 constexpr int SHM_MAX_TB = 9;
@@ -112,6 +117,15 @@ struct ShmHeader {
     // followed by ShmStage[num_stages], ShmOp[num_ops] (8-byte aligned)
 };
 
+// Host-side result of lowering + register-stage planning of one group.
+struct ShmPlan {
+    int n = 0, precision = 0, k = 0, rb = 0, tb = 0;
+    uint64_t shared_mask = 0;
+    ShmHeader hdr;
+    std::vector<ShmStage> stages;
+    std::vector<ShmOp> ops;
+};
+
 }  // namespace hq
 
 struct hq_program_s {
@@ -129,6 +143,11 @@ struct hq_program_s {
     void* d_targs = nullptr; // transpose launch arguments (device)
     void* d_mat = nullptr;   // K x K, column-major, split planes re | im, in state precision
     int mat_qubits = 0;
+    // specialised (JIT) OShareMem kernel; null when the interpreter runs the group
+    hipModule_t jit_module = nullptr;
+    hipFunction_t jit_fn = nullptr;
+    void* d_coef = nullptr;  // runtime coefficients of the specialised kernel
+    hq::ShmPlan* plan = nullptr;  // kept for specialisation (owned)
 };
 
 namespace hq {
@@ -140,4 +159,12 @@ hq_status launch_dense(hq_state s, const void* d_mat, int mat_qubits, void* in, 
 int sharemem_default_rb(int precision, int tile);
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates,
                                  uint64_t shared_mask, int rb, hq_program* out);
+hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gates, uint64_t shared_mask,
+                        int rb, ShmPlan* out);
+// Specialised kernel for a planned group: generate, compile (cached), load.
+hq_status jit_sharemem(const ShmPlan& plan, hq_program p);
+hq_status launch_sharemem_jit(hq_state s, hq_program p);
+hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std::vector<hq_program>& progs,
+                            int threads);
+std::string jit_source(const ShmPlan& plan, std::vector<double>* coef);
 }  // namespace hq

@@ -1,0 +1,494 @@
+// jit.hip -- structure-specialised OShareMem kernels (hiprtc).
+//
+// The interpreter kernel (sharemem.hip) dispatches every op at run time; on
+// gfx950 that costs about 2.6x the FP64 VALU work of the gates themselves
+// (register copies at the dispatch merges, serialised scalar loads).  For big
+// states each group is instead emitted as straight-line HIP: register
+// indices, pair lists, controls, lane/thread/block bit positions and the LDS
+// exchange addresses are literals; X/CNOT become register renaming; the
+// angles of parametric gates stay runtime data (a coefficient array), so
+// circuits that differ only in angles share compiled code.  Sources are
+// compiled with hiprtc for gfx950 in parallel and cached in memory and on
+// disk (HQ_JIT_CACHE, default ~/.cache/hyquas_amd/jit).
+
+#include "hq_ops.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <sys/stat.h>
+#include <thread>
+#include <unordered_map>
+
+namespace hq {
+
+namespace {
+
+std::string lit(double v, bool f32)
+{
+    char b[64];
+    if (f32) snprintf(b, sizeof b, "%.9gf", (double)(float)v);
+    else snprintf(b, sizeof b, "%.17g", v);
+    std::string s = b;
+    if (s.find_first_of(".eEn") == std::string::npos) s.insert(s.size() - (f32 ? 1 : 0), ".0");
+    return "(R)(" + s + ")";
+}
+
+std::string hex64(uint64_t v)
+{
+    char b[32];
+    snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
+    return b;
+}
+
+std::string hex32(uint32_t v)
+{
+    char b[32];
+    snprintf(b, sizeof b, "0x%xu", v);
+    return b;
+}
+
+int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+
+uint32_t swz_of(uint32_t x, const ShmHeader& H, int K)
+{
+    uint32_t f = 0;
+    for (int i = 3; i < K; i++)
+        if ((x >> i) & 1) f ^= H.swz_col[i];
+    return x ^ f;
+}
+
+struct Emitter {
+    const ShmPlan& P;
+    std::vector<double>* coef;
+    std::ostringstream o;
+    int NR, NT, K;
+    bool f32;
+    std::vector<int> name;   // register j is held in variable r<name[j]>
+
+    Emitter(const ShmPlan& p, std::vector<double>* c) : P(p), coef(c)
+    {
+        NR = 1 << p.rb;
+        NT = 1 << p.tb;
+        K = p.k;
+        f32 = p.precision != HQ_C128;
+        name.resize(NR);
+        for (int j = 0; j < NR; j++) name[j] = j;
+    }
+
+    std::string r(int j) const { return "r" + std::to_string(name[j]); }
+
+    // physical offset (as a literal) of register index j in stage s
+    uint64_t reg_off(const ShmStage& st, int j) const
+    {
+        uint64_t off = 0;
+        for (int i = 0; i < P.rb; i++)
+            if ((j >> i) & 1) off |= P.hdr.tile_phys[st.reg_bits[i]];
+        return off;
+    }
+
+    uint32_t reg_tile(const ShmStage& st, int j) const
+    {
+        uint32_t e = 0;
+        for (int i = 0; i < P.rb; i++)
+            if ((j >> i) & 1) e |= 1u << st.reg_bits[i];
+        return e;
+    }
+
+    // lane part of the physical address for stage st (runs of consecutive
+    // bits are moved together)
+    void lane_phys(const ShmStage& st, const char* var)
+    {
+        o << "    unsigned long long " << var << " = 0;\n";
+        for (int i = 0; i < P.tb; i++) {
+            int p = ctz64(P.hdr.tile_phys[st.thr_bits[i]]);
+            int len = 1;
+            while (i + len < P.tb && ctz64(P.hdr.tile_phys[st.thr_bits[i + len]]) == p + len) len++;
+            o << "    " << var << " |= (unsigned long long)((t >> " << i << ") & " << hex32((1u << len) - 1)
+              << ") << " << p << ";\n";
+            i += len - 1;
+        }
+    }
+
+    // swizzled LDS index of the lane part for stage st
+    void lane_lds(const ShmStage& st, const char* var)
+    {
+        o << "    unsigned " << var << " = 0;\n";
+        for (int i = 0; i < P.tb; i++) {
+            uint32_t sw = swz_of(1u << st.thr_bits[i], P.hdr, K);
+            o << "    " << var << " ^= (0u - ((t >> " << i << ") & 1u)) & " << hex32(sw) << ";\n";
+        }
+    }
+
+    // coefficient expressions of op oi (constants or runtime array entries)
+    std::vector<std::string> coeffs(const ShmOp& op)
+    {
+        std::vector<std::string> m(8);
+        if (op.flags & SHM_FLAG_CONST) {
+            for (int i = 0; i < 8; i++) m[i] = lit(op.m[i], f32);
+        } else {
+            size_t base = coef->size();
+            for (int i = 0; i < 8; i++) coef->push_back(op.m[i]);
+            for (int i = 0; i < 8; i++) m[i] = "cf[" + std::to_string(base + i) + "]";
+        }
+        return m;
+    }
+
+    void scale_stmt(int j, const std::string& fr, const std::string& fi)
+    {
+        o << "      { const C v = " << r(j) << "; " << r(j) << ".x = " << fr << " * v.x - " << fi << " * v.y; "
+          << r(j) << ".y = " << fr << " * v.y + " << fi << " * v.x; }\n";
+    }
+
+    void emit_op(const ShmOp& op, bool identity_ok_renaming)
+    {
+        std::vector<std::string> m = coeffs(op);
+        const bool const_op = op.flags & SHM_FLAG_CONST;
+        o << "    {\n";
+        bool blockc = op.cblk != 0;
+        if (blockc) o << "     if ((b & " << hex64(op.cblk) << ") == " << hex64(op.cblk) << ") {\n";
+        // per-lane identity where a thread-bit control is 0
+        std::vector<std::string> mv(8);
+        static const char* ident[8] = {"(R)1", "(R)0", "(R)0", "(R)0", "(R)0", "(R)0", "(R)1", "(R)0"};
+        if (op.cthr) {
+            o << "      const bool pc = (t & " << hex32(op.cthr) << ") == " << hex32(op.cthr) << ";\n";
+            for (int i = 0; i < 8; i++) {
+                o << "      const R m" << i << " = pc ? " << m[i] << " : " << ident[i] << ";\n";
+                mv[i] = "m" + std::to_string(i);
+            }
+        } else {
+            for (int i = 0; i < 8; i++) {
+                if (const_op) {
+                    mv[i] = m[i];
+                } else {
+                    o << "      const R m" << i << " = " << m[i] << ";\n";
+                    mv[i] = "m" + std::to_string(i);
+                }
+            }
+        }
+        const bool rename = identity_ok_renaming && !blockc;
+        if (op.where == SHM_IN_REG) {
+            const int Pb = op.tpos;
+            for (int j = 0; j < NR; j++) {
+                if (j & (1 << Pb)) continue;
+                if ((j & op.creg) != op.creg) continue;
+                const int jh = j | (1 << Pb);
+                switch (op.cls) {
+                case SHM_SWAP:
+                    if (rename) std::swap(name[j], name[jh]);
+                    else o << "      { const C tmp = " << r(j) << "; " << r(j) << " = " << r(jh) << "; " << r(jh) << " = tmp; }\n";
+                    break;
+                case SHM_Y:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = hi.y; " << r(j)
+                      << ".y = -hi.x; " << r(jh) << ".x = -lo.y; " << r(jh) << ".y = lo.x; }\n";
+                    break;
+                case SHM_REAL:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; "
+                      << r(j) << ".x = " << mv[0] << " * lo.x + " << mv[2] << " * hi.x; "
+                      << r(j) << ".y = " << mv[0] << " * lo.y + " << mv[2] << " * hi.y; "
+                      << r(jh) << ".x = " << mv[4] << " * lo.x + " << mv[6] << " * hi.x; "
+                      << r(jh) << ".y = " << mv[4] << " * lo.y + " << mv[6] << " * hi.y; }\n";
+                    break;
+                case SHM_RX:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; "
+                      << r(j) << ".x = " << mv[0] << " * lo.x - " << mv[3] << " * hi.y; "
+                      << r(j) << ".y = " << mv[0] << " * lo.y + " << mv[3] << " * hi.x; "
+                      << r(jh) << ".x = " << mv[6] << " * hi.x - " << mv[5] << " * lo.y; "
+                      << r(jh) << ".y = " << mv[6] << " * hi.y + " << mv[5] << " * lo.x; }\n";
+                    break;
+                case SHM_GEN:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; "
+                      << r(j) << ".x = (" << mv[0] << " * lo.x - " << mv[1] << " * lo.y) + (" << mv[2] << " * hi.x - " << mv[3] << " * hi.y); "
+                      << r(j) << ".y = (" << mv[0] << " * lo.y + " << mv[1] << " * lo.x) + (" << mv[2] << " * hi.y + " << mv[3] << " * hi.x); "
+                      << r(jh) << ".x = (" << mv[4] << " * lo.x - " << mv[5] << " * lo.y) + (" << mv[6] << " * hi.x - " << mv[7] << " * hi.y); "
+                      << r(jh) << ".y = (" << mv[4] << " * lo.y + " << mv[5] << " * lo.x) + (" << mv[6] << " * hi.y + " << mv[7] << " * hi.x); }\n";
+                    break;
+                case SHM_DIAG:
+                    scale_stmt(j, mv[0], mv[1]);
+                    scale_stmt(jh, mv[6], mv[7]);
+                    break;
+                default:  // SHM_PHASE
+                    scale_stmt(jh, mv[6], mv[7]);
+                    break;
+                }
+            }
+        } else {
+            // diagonal on a thread bit (per lane) or a block bit (uniform)
+            if (op.where == SHM_IN_THREAD)
+                o << "      const bool hb = (t >> " << (int)op.tpos << ") & 1u;\n";
+            else
+                o << "      const bool hb = (b >> " << (int)op.tpos << ") & 1ull;\n";
+            o << "      const R fr = hb ? " << mv[6] << " : " << mv[0] << ";\n";
+            o << "      const R fi = hb ? " << mv[7] << " : " << mv[1] << ";\n";
+            for (int j = 0; j < NR; j++) {
+                if ((j & op.creg) != op.creg) continue;
+                scale_stmt(j, "fr", "fi");
+            }
+        }
+        if (blockc) o << "     }\n";
+        o << "    }\n";
+    }
+
+    std::string emit()
+    {
+        const int nst = (int)P.stages.size();
+        o << "typedef " << (f32 ? "float" : "double") << " R;\n";
+        o << "struct __attribute__((aligned(" << (f32 ? 8 : 16) << "))) C { R x, y; };\n";
+        o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") void hq_group(C* __restrict__ psi, const R* __restrict__ cf)\n{\n";
+        if (nst > 1) o << "    __shared__ C lds[" << (1 << K) << "];\n";
+        o << "    const unsigned t = threadIdx.x;\n";
+        o << "    const unsigned long long b = blockIdx.x;\n";
+        o << "    unsigned long long base = 0;\n";
+        for (uint32_t j = 0; j < P.hdr.num_block_bits; j++) {
+            int p = ctz64(P.hdr.block_phys[j]);
+            uint32_t len = 1;
+            while (j + len < P.hdr.num_block_bits && ctz64(P.hdr.block_phys[j + len]) == p + (int)len) len++;
+            o << "    base |= ((b >> " << j << ") & " << hex64((1ull << len) - 1) << ") << " << p << ";\n";
+            j += len - 1;
+        }
+        o << "    C";
+        for (int j = 0; j < NR; j++) o << (j ? ", " : " ") << "r" << j;
+        o << ";\n";
+        {
+            const ShmStage& st = P.stages[0];
+            o << "    {\n";
+            lane_phys(st, "lp");
+            o << "    const C* __restrict__ src = psi + (base + lp);\n";
+            for (int j = 0; j < NR; j++) o << "    r" << j << " = src[" << reg_off(st, j) << "ull];\n";
+            o << "    }\n";
+        }
+        for (int s = 0; s < nst; s++) {
+            const ShmStage& st = P.stages[s];
+            if (s > 0) {
+                const ShmStage& prev = P.stages[s - 1];
+                o << "    {\n";
+                lane_lds(prev, "sw");
+                for (int j = 0; j < NR; j++)
+                    o << "    lds[sw ^ " << hex32(swz_of(reg_tile(prev, j), P.hdr, K)) << "] = " << r(j) << ";\n";
+                o << "    }\n    __syncthreads();\n    {\n";
+                lane_lds(st, "sr");
+                for (int j = 0; j < NR; j++) name[j] = j;
+                for (int j = 0; j < NR; j++)
+                    o << "    r" << j << " = lds[sr ^ " << hex32(swz_of(reg_tile(st, j), P.hdr, K)) << "];\n";
+                o << "    }\n";
+            }
+            for (uint32_t oi = st.op_begin; oi < st.op_end; oi++) emit_op(P.ops[oi], true);
+        }
+        {
+            const ShmStage& st = P.stages[nst - 1];
+            o << "    {\n";
+            lane_phys(st, "lq");
+            o << "    C* __restrict__ dst = psi + (base + lq);\n";
+            for (int j = 0; j < NR; j++) o << "    dst[" << reg_off(st, j) << "ull] = " << r(j) << ";\n";
+            o << "    }\n";
+        }
+        o << "}\n";
+        return o.str();
+    }
+};
+
+// ---- compile cache ---------------------------------------------------------------
+
+uint64_t fnv1a(const std::string& s)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+    return h;
+}
+
+std::mutex g_cache_mu;
+std::unordered_map<std::string, std::vector<char>>& mem_cache()
+{
+    static std::unordered_map<std::string, std::vector<char>> m;
+    return m;
+}
+
+std::string cache_dir()
+{
+    const char* e = getenv("HQ_JIT_CACHE");
+    if (e && *e) return e;
+    const char* x = getenv("XDG_CACHE_HOME");
+    if (x && *x) return std::string(x) + "/hyquas_amd/jit";
+    const char* h = getenv("HOME");
+    if (h && *h) return std::string(h) + "/.cache/hyquas_amd/jit";
+    return "";
+}
+
+void mkdirs(const std::string& path)
+{
+    std::string cur;
+    std::stringstream ss(path);
+    std::string part;
+    if (!path.empty() && path[0] == '/') cur = "/";
+    while (std::getline(ss, part, '/')) {
+        if (part.empty()) continue;
+        cur += part + "/";
+        mkdir(cur.c_str(), 0755);
+    }
+}
+
+bool read_file(const std::string& p, std::string* out)
+{
+    std::ifstream f(p, std::ios::binary);
+    if (!f) return false;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    *out = ss.str();
+    return true;
+}
+
+const char* kSrcHeader = "// hyquas_amd group kernel v1\n";
+
+hq_status compile_source(const std::string& src, std::vector<char>* code, std::string* err)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = mem_cache().find(src);
+        if (it != mem_cache().end()) { *code = it->second; return HQ_OK; }
+    }
+    const std::string dir = cache_dir();
+    char key[40];
+    snprintf(key, sizeof key, "%016llx_%zu", (unsigned long long)fnv1a(src), src.size());
+    if (!dir.empty()) {
+        std::string s2, c2;
+        if (read_file(dir + "/" + key + ".hip", &s2) && s2 == src && read_file(dir + "/" + key + ".co", &c2) &&
+            !c2.empty()) {
+            code->assign(c2.begin(), c2.end());
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            mem_cache()[src] = *code;
+            return HQ_OK;
+        }
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "hq_group.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        *err = "hiprtcCreateProgram failed";
+        return HQ_ERR_HIP;
+    }
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=fast", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        *err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + "\n" + log.substr(0, 2000);
+        hiprtcDestroyProgram(&prog);
+        return HQ_ERR_HIP;
+    }
+    size_t sz = 0;
+    hiprtcGetCodeSize(prog, &sz);
+    code->resize(sz);
+    hiprtcGetCode(prog, code->data());
+    hiprtcDestroyProgram(&prog);
+    if (!dir.empty()) {
+        mkdirs(dir);
+        std::string tmp = dir + "/" + key + ".tmp" + std::to_string((unsigned long long)(uintptr_t)code);
+        {
+            std::ofstream f(tmp + ".co", std::ios::binary);
+            f.write(code->data(), (std::streamsize)code->size());
+        }
+        {
+            std::ofstream f(tmp + ".hip", std::ios::binary);
+            f << src;
+        }
+        rename((tmp + ".co").c_str(), (dir + "/" + key + ".co").c_str());
+        rename((tmp + ".hip").c_str(), (dir + "/" + key + ".hip").c_str());
+    }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    mem_cache()[src] = *code;
+    return HQ_OK;
+}
+
+}  // namespace
+
+std::string jit_source(const ShmPlan& plan, std::vector<double>* coef)
+{
+    Emitter e(plan, coef);
+    return std::string(kSrcHeader) + e.emit();
+}
+
+static hq_status load_program(const ShmPlan& plan, const std::vector<char>& code, const std::vector<double>& coef,
+                              hq_program p)
+{
+    HQ_HIP(hipModuleLoadData(&p->jit_module, code.data()));
+    HQ_HIP(hipModuleGetFunction(&p->jit_fn, p->jit_module, "hq_group"));
+    size_t rbytes = plan.precision == HQ_C128 ? 8 : 4;
+    size_t n = std::max<size_t>(coef.size(), 1);
+    HQ_HIP(hipMalloc(&p->d_coef, n * rbytes));
+    if (!coef.empty()) {
+        if (rbytes == 8) {
+            HQ_HIP(hipMemcpy(p->d_coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(coef.begin(), coef.end());
+            HQ_HIP(hipMemcpy(p->d_coef, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
+    return HQ_OK;
+}
+
+hq_status jit_sharemem(const ShmPlan& plan, hq_program p)
+{
+    std::vector<double> coef;
+    std::string src = jit_source(plan, &coef);
+    std::vector<char> code;
+    std::string err;
+    hq_status st = compile_source(src, &code, &err);
+    if (st != HQ_OK) { set_error(err); return st; }
+    return load_program(plan, code, coef, p);
+}
+
+// Compile many plans in parallel (hiprtc programs are independent).
+hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std::vector<hq_program>& progs,
+                            int threads)
+{
+    const size_t n = plans.size();
+    std::vector<std::string> srcs(n);
+    std::vector<std::vector<double>> coefs(n);
+    for (size_t i = 0; i < n; i++) srcs[i] = jit_source(*plans[i], &coefs[i]);
+    std::vector<std::vector<char>> codes(n);
+    std::vector<hq_status> sts(n, HQ_OK);
+    std::vector<std::string> errs(n);
+    // warm the compiler once before fanning out
+    if (n > 0) sts[0] = compile_source(srcs[0], &codes[0], &errs[0]);
+    std::mutex mu;
+    size_t next = 1;
+    auto worker = [&]() {
+        while (true) {
+            size_t i;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (next >= n) return;
+                i = next++;
+            }
+            sts[i] = compile_source(srcs[i], &codes[i], &errs[i]);
+        }
+    };
+    int nt = std::max(1, std::min<int>(threads, (int)n - 1));
+    std::vector<std::thread> pool;
+    for (int i = 0; i < nt && n > 1; i++) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+    for (size_t i = 0; i < n; i++) {
+        if (sts[i] != HQ_OK) { set_error(errs[i]); return sts[i]; }
+        hq_status st = load_program(*plans[i], codes[i], coefs[i], progs[i]);
+        if (st != HQ_OK) return st;
+    }
+    return HQ_OK;
+}
+
+hq_status launch_sharemem_jit(hq_state s, hq_program p)
+{
+    void* psi = s->data;
+    void* cf = p->d_coef;
+    void* args[] = {&psi, &cf};
+    unsigned blocks = (unsigned)((uint64_t)1 << (p->n - p->tile));
+    HQ_HIP(hipModuleLaunchKernel(p->jit_fn, blocks, 1, 1, 1u << p->tb, 1, 1, 0, s->stream, args, nullptr));
+    return HQ_OK;
+}
+
+}  // namespace hq

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 using namespace hq;
 
@@ -100,7 +101,7 @@ hq_status hq_program_launch(hq_state s, hq_program p)
     if (p->kind == 1) {
         st = prof_begin(s, 1, &slot);
         if (st != HQ_OK) return st;
-        st = launch_sharemem(s, p);
+        st = p->jit_fn ? launch_sharemem_jit(s, p) : launch_sharemem(s, p);
         if (st != HQ_OK) return st;
         return prof_end(s, slot);
     }
@@ -133,7 +134,26 @@ hq_status hq_program_destroy(hq_program p)
     if (p->d_prog) (void)hipFree(p->d_prog);
     if (p->d_targs) (void)hipFree(p->d_targs);
     if (p->d_mat) (void)hipFree(p->d_mat);
+    if (p->d_coef) (void)hipFree(p->d_coef);
+    if (p->jit_module) (void)hipModuleUnload(p->jit_module);
+    delete p->plan;
     delete p;
+    return HQ_OK;
+}
+
+hq_status hq_program_specialize(hq_program p)
+{
+    HQ_CHECK(p, HQ_ERR_ARG, "hq_program_specialize: null program");
+    if (p->kind != 1) return HQ_OK;   // TransMM programs have nothing to specialise
+    if (p->jit_fn) return HQ_OK;
+    HQ_CHECK(p->plan, HQ_ERR_STATE, "hq_program_specialize: program has no plan");
+    return jit_sharemem(*p->plan, p);
+}
+
+hq_status hq_program_is_specialized(hq_program p, int* yes)
+{
+    HQ_CHECK(p && yes, HQ_ERR_ARG, "hq_program_is_specialized: null argument");
+    *yes = p->jit_fn != nullptr;
     return HQ_OK;
 }
 
@@ -184,3 +204,29 @@ hq_status hq_apply_dense(hq_state s, const double* mat, int mat_qubits)
 }
 
 }  // extern "C"
+
+namespace hq {
+
+hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
+                                  std::vector<hq_program>* out)
+{
+    out->assign(reqs.size(), nullptr);
+    for (size_t i = 0; i < reqs.size(); i++) {
+        hq_status st = build_sharemem_program(s, reqs[i].gates.data(), (int)reqs[i].gates.size(), reqs[i].mask,
+                                              reg_qubits, &(*out)[i]);
+        if (st != HQ_OK) {
+            for (auto p : *out) hq_program_destroy(p);
+            out->assign(reqs.size(), nullptr);
+            return st;
+        }
+    }
+    if (!jit || reqs.empty()) return HQ_OK;
+    std::vector<const ShmPlan*> plans;
+    for (auto p : *out) plans.push_back(p->plan);
+    unsigned hw = std::thread::hardware_concurrency();
+    const char* e = getenv("OMP_NUM_THREADS");
+    int threads = e && atoi(e) > 0 ? atoi(e) : (int)(hw ? hw : 4);
+    return jit_sharemem_many(plans, *out, std::min(threads, 16));
+}
+
+}  // namespace hq

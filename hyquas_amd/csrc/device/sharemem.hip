@@ -284,19 +284,17 @@ int rank3(const std::vector<uint32_t>& v)
 
 }  // namespace
 
-hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
-                                 hq_program* out)
+hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
+                        ShmPlan* plan)
 {
-    HQ_CHECK(s && out, HQ_ERR_ARG, "sharemem: null argument");
-    *out = nullptr;
-    const int n = s->n;
     HQ_CHECK(num_gates >= 0 && (num_gates == 0 || gates), HQ_ERR_ARG, "sharemem: bad gate array");
+    const size_t amp_bytes = precision == HQ_C128 ? 16 : 8;
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     HQ_CHECK((shared_mask & ~all) == 0, HQ_ERR_ARG, "sharemem: shared mask has bits beyond the state");
     const int k = __builtin_popcountll(shared_mask);
     HQ_CHECK(k >= 7 && k <= SHM_MAX_TILE, HQ_ERR_ARG, "sharemem: tile must have 7..14 qubits");
-    HQ_CHECK(((size_t)1 << k) * s->amp_bytes <= 160 * 1024, HQ_ERR_ARG, "sharemem: tile exceeds the 160 KiB LDS");
-    const int rb = rb_req > 0 ? rb_req : sharemem_default_rb(s->precision, k);
+    HQ_CHECK(((size_t)1 << k) * amp_bytes <= 160 * 1024, HQ_ERR_ARG, "sharemem: tile exceeds the 160 KiB LDS");
+    const int rb = rb_req > 0 ? rb_req : sharemem_default_rb(precision, k);
     const int tb = k - rb;
     HQ_CHECK(shape_supported(rb, tb), HQ_ERR_UNSUPPORTED, "sharemem: unsupported register/thread split");
 
@@ -308,6 +306,7 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
     }
 
     std::vector<Lowered> low;
+    std::vector<int> low_type;
     low.reserve(num_gates);
     for (int i = 0; i < num_gates; i++) {
         const hq_gate& g = gates[i];
@@ -330,6 +329,7 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
         }
         memcpy(L.m, g.mat, sizeof(L.m));
         low.push_back(L);
+        low_type.push_back(g.type);
     }
 
     // greedy stage planning: take gates in order; a stage has at most RB
@@ -421,6 +421,18 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
                 }
             }
             memcpy(o.m, L.m, sizeof(o.m));
+            // a thread-bit control makes the op the identity on some lanes:
+            // use an update form that can express the identity
+            if (o.cthr && o.cls == SHM_SWAP) o.cls = SHM_REAL;
+            if (o.cthr && o.cls == SHM_Y) o.cls = SHM_RX;
+            switch (low_type[idx]) {
+            case HQ_CCX: case HQ_CNOT: case HQ_CY: case HQ_CZ: case HQ_H: case HQ_X: case HQ_Y: case HQ_Z:
+            case HQ_S: case HQ_SDG: case HQ_T: case HQ_TDG: case HQ_ID: case HQ_GII: case HQ_GZZ:
+                o.flags |= SHM_FLAG_CONST;
+                break;
+            default:
+                break;
+            }
             ops.push_back(o);
         }
         d.op_end = (uint32_t)ops.size();
@@ -435,27 +447,51 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
     hdr.num_stages = (uint32_t)dst.size();
     hdr.num_ops = (uint32_t)ops.size();
 
+    plan->n = n;
+    plan->precision = precision;
+    plan->k = k;
+    plan->rb = rb;
+    plan->tb = tb;
+    plan->shared_mask = shared_mask;
+    plan->hdr = hdr;
+    plan->stages = std::move(dst);
+    plan->ops = std::move(ops);
+    return HQ_OK;
+}
+
+hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
+                                 hq_program* out)
+{
+    HQ_CHECK(s && out, HQ_ERR_ARG, "sharemem: null argument");
+    *out = nullptr;
+    ShmPlan plan;
+    hq_status st = plan_sharemem(s->n, s->precision, gates, num_gates, shared_mask, rb_req, &plan);
+    if (st != HQ_OK) return st;
+    const auto& dst = plan.stages;
+    const auto& ops = plan.ops;
     size_t bytes = sizeof(ShmHeader) + dst.size() * sizeof(ShmStage) + ops.size() * sizeof(ShmOp);
     std::vector<char> blob(bytes);
-    memcpy(blob.data(), &hdr, sizeof(hdr));
-    memcpy(blob.data() + sizeof(hdr), dst.data(), dst.size() * sizeof(ShmStage));
+    memcpy(blob.data(), &plan.hdr, sizeof(ShmHeader));
+    memcpy(blob.data() + sizeof(ShmHeader), dst.data(), dst.size() * sizeof(ShmStage));
     if (!ops.empty())
-        memcpy(blob.data() + sizeof(hdr) + dst.size() * sizeof(ShmStage), ops.data(), ops.size() * sizeof(ShmOp));
+        memcpy(blob.data() + sizeof(ShmHeader) + dst.size() * sizeof(ShmStage), ops.data(), ops.size() * sizeof(ShmOp));
 
     hq_program p = new hq_program_s();
     p->kind = 1;
-    p->n = n;
+    p->n = plan.n;
     p->precision = s->precision;
-    p->tile = k;
-    p->rb = rb;
-    p->tb = tb;
+    p->tile = plan.k;
+    p->rb = plan.rb;
+    p->tb = plan.tb;
     p->num_stages = (int)dst.size();
     p->num_ops = (int)ops.size();
     p->shared_mask = shared_mask;
     p->prog_bytes = bytes;
+    p->plan = new ShmPlan(std::move(plan));
     hipError_t e = hipMalloc(&p->d_prog, bytes);
     if (e != hipSuccess) {
         set_error(std::string("sharemem: hipMalloc: ") + hipGetErrorString(e));
+        delete p->plan;
         delete p;
         return HQ_ERR_OOM;
     }
@@ -463,6 +499,7 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
     if (e != hipSuccess) {
         set_error(std::string("sharemem: hipMemcpy: ") + hipGetErrorString(e));
         (void)hipFree(p->d_prog);
+        delete p->plan;
         delete p;
         return HQ_ERR_HIP;
     }

@@ -163,6 +163,9 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
     } else if (k == "window") {
         HQ_CARG(v >= 1 && v <= 4096, "window must be in [1, 4096]");
         C.params.window = (int)v;
+    } else if (k == "jit") {
+        HQ_CARG(v >= -1 && v <= 1, "jit must be -1 (auto), 0 or 1");
+        C.exec.jit = (int)v;
     } else if (k == "profile_events") {
         C.profileEvents = v != 0;
     } else {

@@ -30,6 +30,8 @@ static const int kMinTile = 7;
 hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& p)
 {
     State st(n);
+    std::vector<hq::SharememRequest> reqs;
+    std::vector<GateGroup*> owners;
     for (auto& lg : sch.localGroups) {
         for (auto& gg : lg.fullGroups) {
             if (gg.program) { hq_program_destroy(gg.program); gg.program = nullptr; }
@@ -39,28 +41,24 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                     hq::set_error("executor: group has more related qubits than the tile");
                     return HQ_ERR_ARG;
                 }
+                gg.state = st;
+                gg.numStages = 0;   // tiles below kMinTile: plain launches at run time
                 if (k >= kMinTile) {
                     qindex rel = fillRelated(gg.relatedQubits, st, k);
-                    uint64_t mask = 0;
+                    hq::SharememRequest r;
                     for (int q = 0; q < n; q++)
-                        if ((rel >> q) & 1) mask |= 1ull << st.pos[q];
-                    std::vector<hq_gate> hg;
-                    hg.reserve(gg.gates.size());
+                        if ((rel >> q) & 1) r.mask |= 1ull << st.pos[q];
+                    r.gates.reserve(gg.gates.size());
                     for (const Gate& g : gg.gates) {
                         hq_gate h = g.toHQ();
                         h.target = st.pos[g.targetQubit];
                         h.control = g.controlQubit >= 0 ? st.pos[g.controlQubit] : -1;
                         h.control2 = g.controlQubit2 >= 0 ? st.pos[g.controlQubit2] : -1;
-                        hg.push_back(h);
+                        r.gates.push_back(h);
                     }
-                    hq_status rc = hq::build_sharemem_program(s, hg.data(), (int)hg.size(), mask, p.regQubits,
-                                                              &gg.program);
-                    if (rc != HQ_OK) return rc;
-                    hq_program_info(gg.program, nullptr, &gg.numStages, nullptr);
-                } else {
-                    gg.numStages = 0;   // plain launches at run time
+                    reqs.push_back(std::move(r));
+                    owners.push_back(&gg);
                 }
-                gg.state = st;
             } else if (gg.backend == Backend::BLAS) {
                 gg.initCPUMatrix();
                 const size_t K = (size_t)1 << gg.matQubit;
@@ -71,6 +69,14 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                 st = gg.state;
             }
         }
+    }
+    const bool jit = p.jit == 1 || (p.jit < 0 && n >= p.jitMinQubits);
+    std::vector<hq_program> progs;
+    hq_status rc = hq::build_sharemem_programs(s, reqs, p.regQubits, jit, &progs);
+    if (rc != HQ_OK) return rc;
+    for (size_t i = 0; i < owners.size(); i++) {
+        owners[i]->program = progs[i];
+        hq_program_info(progs[i], nullptr, &owners[i]->numStages, nullptr);
     }
     return HQ_OK;
 }

@@ -9,6 +9,8 @@
 struct ExecParams {
     int localSize = 12;   // OShareMem tile qubits (fillRelatedQubits target)
     int regQubits = 0;    // register qubits per thread (0: default for the tile)
+    int jit = -1;         // specialised kernels: 0 off, 1 on, -1 from jitMinQubits
+    int jitMinQubits = 22;
 };
 
 class Executor {

@@ -5,6 +5,7 @@
 #include "hyquas_amd.h"
 
 #include <string>
+#include <vector>
 
 namespace hq {
 void set_error(const std::string& msg);
@@ -12,4 +13,13 @@ void set_error(const std::string& msg);
 // (reg_qubits = 0: default for the tile size).
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask,
                                  int reg_qubits, hq_program* out);
+
+// Many OShareMem programs at once; with jit, every group also gets a
+// structure-specialised kernel, compiled in parallel.
+struct SharememRequest {
+    std::vector<hq_gate> gates;
+    uint64_t mask = 0;
+};
+hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
+                                  std::vector<hq_program>* out);
 }  // namespace hq

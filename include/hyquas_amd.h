@@ -156,6 +156,16 @@ hq_status hq_program_create_blas(hq_state s, const int *perm, const double *mat_
 hq_status hq_program_launch(hq_state s, hq_program p);
 hq_status hq_program_destroy(hq_program p);
 
+/* Replace the interpreter of an OShareMem program by a kernel specialised to
+ * its structure (register/thread/block bit layout, pairs, controls), emitted
+ * as HIP source and compiled for gfx950 with hiprtc; parametric angles stay
+ * runtime data.  Compiled code is cached (HQ_JIT_CACHE, default
+ * ~/.cache/hyquas_amd/jit).  No-op for TransMM programs.  This build's
+ * replacement of the compile-time specialisation the reference gets from
+ * its per-type kernel switch (src/kernelOpt.cu:214-386). */
+hq_status hq_program_specialize(hq_program p);
+hq_status hq_program_is_specialized(hq_program p, int *yes);
+
 /* Describe a compiled OShareMem program: tile qubits, register stages and
  * lowered op count (for tests and the cost model). */
 hq_status hq_program_info(hq_program p, int *tile_qubits, int *num_stages, int *num_ops);
@@ -215,6 +225,8 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *   "min_mat"     MIN_MAT_SIZE (default 4)
  *   "reg_qubits"  register qubits per thread in OShareMem (0 = auto)
  *   "window"      gates considered per group (default 512)
+ *   "jit"         specialised OShareMem kernels: 0 off, 1 on, -1 auto
+ *                 (default: on from 22 qubits)
  *   "profile_events" 1 = time every launch with HIP events
  * Must be set before compile. */
 hq_status hq_circuit_set_option(hq_circuit c, const char *key, int64_t value);

@@ -67,6 +67,17 @@ def test_random_layers_serial_plain_20q(gpu):
     assert rel_err(c.logical_state(), want) <= 1e-12
 
 
+@pytest.mark.parametrize("jit", [0, 1])
+def test_random_layers_jit_and_interpreter(gpu, jit):
+    n = 17
+    c = gpu.Circuit(n).add_random_layers(10, 5)
+    c.set_option("jit", jit)
+    c.compile()
+    c.run(copy_back=False, destroy=False)
+    want = oracle.run_plain(to_oracle(random_layers(n, 10, 5)), n)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+
+
 def test_random_layers_full_vector(gpu):
     n = 18
     c = gpu.Circuit(n).add_random_layers(12, 3)

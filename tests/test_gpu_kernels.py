@@ -78,6 +78,27 @@ def test_sharemem_matches_oracle(gpu, n, k, seed):
     assert rel_err(s.download(), want) <= 1e-12
 
 
+@pytest.mark.parametrize("n,k,seed,prec", [(12, 10, 21, 0), (14, 12, 22, 0), (16, 13, 23, 0), (15, 12, 24, 1),
+                                            (16, 13, 25, 1)])
+def test_sharemem_specialized_matches_oracle(gpu, n, k, seed, prec):
+    """The hiprtc-specialised kernel of a group gives the interpreter's result."""
+    rs = np.random.default_rng(seed)
+    low = list(range(3))
+    rest = rs.choice(np.arange(3, n), k - 3, replace=False)
+    mask = 0
+    for q in low + [int(x) for x in rest]:
+        mask |= 1 << q
+    g = _group_for_mask(n, mask, 150, seed)
+    psi0 = _rand_state(n, seed)
+    want = oracle.run_plain(g, n, psi0)
+    s = gpu.State(n, prec)
+    s.upload(psi0.astype(s.dtype))
+    p = s.program_sharemem(g, mask).specialize()
+    assert p.is_specialized()
+    s.launch(p)
+    assert rel_err(s.download(), want) <= (1e-12 if prec == 0 else 2e-6)
+
+
 def test_sharemem_c64(gpu):
     n, mask = 15, (1 << 13) - 1
     g = _group_for_mask(n, mask, 100, 9)
