@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--coalesce", type=int, default=0, help="low qubits always in the tile (0: default)")
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-pass", action="store_true",
+                    help="before timing, run one zero-gate OShareMem pass (a copy of known size, "
+                         "2 * 2^n * amplitude bytes) to calibrate HBM counters")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -118,6 +121,14 @@ def main():
 
     for _ in range(a.warmup):
         c.run(copy_back=False, destroy=False)
+
+    if a.copy_pass:
+        st = c.state_view(prec)
+        mask = (1 << min(12, n)) - 1
+        prog = st.program_sharemem(hq.gates_to_array([]), mask)
+        st.launch(prog)
+        st.sync()
+        prog.close()
 
     launches = []
     t0 = time.perf_counter()

@@ -492,3 +492,27 @@ hq_status launch_sharemem_jit(hq_state s, hq_program p)
 }
 
 }  // namespace hq
+
+extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gates, int num_gates,
+                                        uint64_t shared_mask, int reg_qubits, char* buf, size_t cap, size_t* len,
+                                        double* coef, size_t coef_cap, size_t* num_coef, int* shape)
+{
+    HQ_CHECK(n >= 1 && n <= 36, HQ_ERR_ARG, "hq_sharemem_source: num_qubits must be in [1, 36]");
+    HQ_CHECK(precision == HQ_C128 || precision == HQ_C64, HQ_ERR_ARG, "hq_sharemem_source: bad precision");
+    hq::ShmPlan plan;
+    hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan);
+    if (st != HQ_OK) return st;
+    std::vector<double> cf;
+    std::string src = hq::jit_source(plan, &cf);
+    if (len) *len = src.size();
+    if (buf && cap) {
+        size_t k = std::min(src.size(), cap - 1);
+        memcpy(buf, src.data(), k);
+        buf[k] = 0;
+    }
+    if (num_coef) *num_coef = cf.size();
+    if (coef)
+        for (size_t i = 0; i < cf.size() && i < coef_cap; i++) coef[i] = cf[i];
+    if (shape) { shape[0] = plan.k; shape[1] = plan.rb; shape[2] = plan.tb; }
+    return HQ_OK;
+}

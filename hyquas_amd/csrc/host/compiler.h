@@ -10,7 +10,7 @@ struct CompilerParams {
     int mode = HQ_MODE_MIX;   // HQ_MODE_GROUP / HQ_MODE_BLAS / HQ_MODE_MIX
     int precision = HQ_C128;
     int localSize = 12;       // OShareMem tile qubits (LOCAL_QUBIT_SIZE)
-    int coalesce = 3;         // COALESCE_GLOBAL: lowest physical qubits always in the tile
+    int coalesce = 4;         // COALESCE_GLOBAL: lowest physical qubits always in the tile (reference: 3)
     int minMat = 4;           // MIN_MAT_SIZE
     int blasMat = 6;          // BLAS_MAT_LIMIT (HQ_MODE_BLAS)
     int matLo = 4, matHi = 8; // mix mode tries matLo <= matSize < matHi

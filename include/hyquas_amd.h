@@ -166,6 +166,15 @@ hq_status hq_program_destroy(hq_program p);
 hq_status hq_program_specialize(hq_program p);
 hq_status hq_program_is_specialized(hq_program p, int *yes);
 
+/* Host only (no device needed): plan an OShareMem group and emit the HIP
+ * source of its specialised kernel (extern "C" hq_group(C *psi, const R *cf),
+ * one block of 2^thread_qubits threads per tile, 2^(n - tile) blocks).
+ * coef receives the runtime coefficient array (cf), shape[0..2] the tile,
+ * register and thread qubit counts.  For inspection, caching and CPU tests. */
+hq_status hq_sharemem_source(int num_qubits, int precision, const hq_gate *gates, int num_gates,
+                             uint64_t shared_mask, int reg_qubits, char *buf, size_t cap, size_t *len,
+                             double *coef, size_t coef_cap, size_t *num_coef, int *shape);
+
 /* Describe a compiled OShareMem program: tile qubits, register stages and
  * lowered op count (for tests and the cost model). */
 hq_status hq_program_info(hq_program p, int *tile_qubits, int *num_stages, int *num_ops);
@@ -219,7 +228,7 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *   "profile"     0 = gfx950 parameters (default), 1 = reference parameters
  *                 (LOCAL_QUBIT_SIZE 10, COALESCE 3, V100 cost tables)
  *   "share_qubits" OShareMem tile qubits (default 12; 6..14)
- *   "coalesce"    COALESCE_GLOBAL (default 3)
+ *   "coalesce"    COALESCE_GLOBAL (default 4; the reference uses 3)
  *   "mat_min", "mat_max"  TransMM group qubit range in mix mode (4, 7)
  *   "blas_mat"    BLAS_MAT_LIMIT for HQ_MODE_BLAS (default 6)
  *   "min_mat"     MIN_MAT_SIZE (default 4)

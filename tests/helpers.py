@@ -113,7 +113,7 @@ def random_mixed(n, m, seed, ccx=True):
     out = []
     for _ in range(m):
         r = rs.random()
-        if r < 0.55:
+        if r < 0.55 or n < 2:
             t, npar = ONE_Q[rs.integers(len(ONE_Q))]
             q = int(rs.integers(n))
             out.append((t, q, -1, tuple(rs.uniform(0, 2 * np.pi, npar))))
@@ -131,3 +131,23 @@ def rel_err(a, b):
     a = np.asarray(a, dtype=np.complex128)
     b = np.asarray(b, dtype=np.complex128)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def rand_state(n, seed, dtype=np.complex128):
+    rs = np.random.default_rng(seed)
+    v = rs.normal(size=1 << n) + 1j * rs.normal(size=1 << n)
+    return (v / np.linalg.norm(v)).astype(dtype)
+
+
+def group_for_mask(n, mask, m, seed):
+    """Random gates (all types) whose non-diagonal targets lie in the tile `mask`."""
+    rs = np.random.default_rng(seed)
+    tile = [q for q in range(n) if (mask >> q) & 1]
+    spec = []
+    for s in random_mixed(n, m, seed):
+        t, tgt, c = s[0], s[1], s[2]
+        if not oracle.lib().ora_is_diagonal(t) and tgt not in tile:
+            new_t = int(rs.choice([q for q in tile if q != c and (len(s) < 5 or q != s[4])]))
+            s = (t, new_t) + tuple(s[2:])
+        spec.append(s)
+    return to_oracle(spec)

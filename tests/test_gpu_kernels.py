@@ -7,15 +7,10 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import CNOT, CZ, H, RX, RZ, T, U3, rel_err, random_mixed, to_oracle
+from helpers import H, group_for_mask as _group_for_mask, rand_state as _rand_state, rel_err, random_mixed, \
+    to_oracle
 
 pytestmark = pytest.mark.gpu
-
-
-def _rand_state(n, seed, dtype=np.complex128):
-    rs = np.random.default_rng(seed)
-    v = rs.normal(size=1 << n) + 1j * rs.normal(size=1 << n)
-    return (v / np.linalg.norm(v)).astype(dtype)
 
 
 @pytest.mark.parametrize("n", [3, 8, 13])
@@ -40,20 +35,6 @@ def test_plain_c64(gpu):
     s.upload(psi0.astype(np.complex64))
     s.apply_plain(g)
     assert rel_err(s.download(), want) <= 2e-6
-
-
-def _group_for_mask(n, mask, m, seed):
-    """Random gates whose non-diagonal targets lie in the tile (mask)."""
-    rs = np.random.default_rng(seed)
-    tile = [q for q in range(n) if (mask >> q) & 1]
-    spec = []
-    for s in random_mixed(n, m, seed):
-        t, tgt, c = s[0], s[1], s[2]
-        if not oracle.lib().ora_is_diagonal(t) and tgt not in tile:
-            new_t = int(rs.choice([q for q in tile if q != c and (len(s) < 5 or q != s[4])]))
-            s = (t, new_t) + tuple(s[2:])
-        spec.append(s)
-    return to_oracle(spec)
 
 
 @pytest.mark.parametrize("n,k,seed", [(7, 7, 0), (10, 9, 1), (12, 10, 2), (14, 12, 3), (15, 12, 4), (16, 13, 5)])

@@ -1,0 +1,161 @@
+"""Host-side logic of the product against the oracle, CPU only: gate
+factories, workload generators, the partitioner (reference parameters give
+the reference grouping; gfx950 parameters keep the invariants), the qasm
+reader and the error paths of the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (CCX, CNOT, H, RX, hea_layers, random_layers, random_mixed, to_oracle)
+from test_oracle import QASM, NPAR, _check_partition_invariants
+
+ALL_TYPES = list(range(22)) + [23, 24, 25, 26, 27]
+CTRL = {0: 2, 1: 1, 2: 1, 3: 1, 4: 1, 5: 1, 6: 1, 7: 1}
+PARAMS = dict(NPAR)
+PARAMS.update({26: 2, 27: 2})
+
+
+@pytest.mark.parametrize("t", [x for x in ALL_TYPES if x != 22])
+def test_gate_factories_match_oracle(hq, t):
+    rs = np.random.default_rng(t)
+    p = tuple(rs.uniform(-3, 3, PARAMS.get(t, 0)))
+    nc = CTRL.get(t, 0)
+    c1, c2 = (1 if nc >= 1 else -1), (2 if nc == 2 else -1)
+    g = hq.Gate.make(t, c2, c1, 0, p)
+    o = oracle.gate(t, 0, c1, c2, p)
+    assert list(g.rec.mat) == list(o["mat"])
+    assert (g.rec.type, g.rec.target, g.rec.control, g.rec.control2) == (t, 0, c1, c2)
+
+
+def test_gate_factory_errors(hq):
+    with pytest.raises(hq.HQError):
+        hq.Gate.make(RX, -1, -1, 0, ())
+    with pytest.raises(hq.HQError):
+        hq.Gate.make(CNOT, -1, -1, 0, ())
+    with pytest.raises(hq.HQError):
+        hq.Gate.make(22, -1, -1, 0, ())
+
+
+def _spec_of(c):
+    return [(r.type, r.target, r.control, r.control2, tuple(r.mat)) for r in c.gates()]
+
+
+@pytest.mark.parametrize("n,layers,seed", [(5, 3, 0), (20, 20, 0), (30, 2, 7), (11, 4, 123)])
+def test_random_layer_generator(hq, n, layers, seed):
+    c = hq.Circuit(n).add_random_layers(layers, seed)
+    want = to_oracle(random_layers(n, layers, seed))
+    got = c.gate_array()
+    assert len(got) == len(want) == layers * (n + n // 2)
+    for a, b in zip(got, want):
+        assert (a["type"], a["target"], a["control"]) == (b["type"], b["target"], b["control"])
+        assert np.array_equal(a["mat"], b["mat"])
+
+
+def test_hea_generator(hq):
+    n, layers = 13, 5
+    got = hq.Circuit(n).add_hea_layers(layers, 9).gate_array()
+    want = to_oracle(hea_layers(n, layers, 9))
+    assert len(got) == len(want) == (3 * n - 1) * layers
+    for a, b in zip(got, want):
+        assert (a["type"], a["target"], a["control"]) == (b["type"], b["target"], b["control"])
+        assert np.array_equal(a["mat"], b["mat"])
+
+
+def _product_partition(hq, n, spec, mode, reference=True, **opts):
+    c = hq.Circuit(n)
+    for s in spec:
+        c.addGate(hq.Gate.make(s[0], s[4] if len(s) > 4 else -1, s[2], s[1], s[3] if len(s) > 3 else ()))
+    if reference:
+        c.set_option("profile", 1)
+    c.set_option("mode", mode)
+    for k, v in opts.items():
+        c.set_option(k, v)
+    c.partition()
+    order = c.schedule_order()
+    groups, k = [], 0
+    for g in c.schedule():
+        groups.append(dict(backend=g["backend"], gates=order[k:k + g["num_gates"]], related=g["related"],
+                           mat_qubit=g["mat_qubit"]))
+        k += g["num_gates"]
+    return groups, c.final_pos()
+
+
+@pytest.mark.parametrize("mode", [1, 3, 4])
+@pytest.mark.parametrize("n,seed,kind", [(12, 0, "layers"), (16, 1, "mixed"), (20, 2, "layers"), (28, 3, "mixed"),
+                                         (28, 4, "layers"), (9, 5, "mixed")])
+def test_partition_matches_reference_restatement(hq, mode, n, seed, kind):
+    spec = random_layers(n, 8, seed) if kind == "layers" else random_mixed(n, 300, seed, ccx=False)
+    og, ofpos, _ = oracle.partition(to_oracle(spec), n, oracle.partition_params(n, mode))
+    pg, pfpos = _product_partition(hq, n, spec, mode)
+    assert [(g["backend"], g["gates"], g["related"], g["mat_qubit"]) for g in og] == \
+           [(g["backend"], g["gates"], g["related"], g["mat_qubit"]) for g in pg]
+    assert ofpos == pfpos
+
+
+def test_partition_hand_traced_cases_in_product(hq):
+    n = 12
+    spec = [(H, q, -1, ()) for q in range(n)] + [(CNOT, 11, 0, ())]
+    pg, _ = _product_partition(hq, n, spec, 1)
+    assert [g["gates"] for g in pg] == [[0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 12], [9, 10]]
+    n = 8
+    spec = [(H, q, -1, ()) for q in range(n)] + [(CNOT, 1, 0, ())]
+    pg, fpos = _product_partition(hq, n, spec, 3)
+    assert [g["gates"] for g in pg] == [[0, 1, 2, 3, 4, 5, 8], [6, 7]]
+    assert fpos == [6, 7, 2, 3, 4, 5, 0, 1]
+
+
+@pytest.mark.parametrize("share", [10, 12, 13])
+def test_gfx950_partition_invariants(hq, share):
+    n = 24
+    spec = random_mixed(n, 400, 11, ccx=True)
+    pg, _ = _product_partition(hq, n, spec, 4, reference=False, share_qubits=share)
+    _check_partition_invariants(spec, n, pg, share, 7)
+
+
+def test_ccx_second_control_is_a_dependency(hq):
+    """H(2) before CCX(0, 2 -> 1) must stay before it (the reference's
+    getGroupOpt ignores controlQubit2 and may reorder them)."""
+    n = 14
+    spec = [(RX, 2, -1, (0.3,))] * 1 + [(CCX, 1, 0, (), 2), (H, 2, -1, ())]
+    spec = [(H, q, -1, ()) for q in range(4, n)] + spec
+    pg, _ = _product_partition(hq, n, spec, 1)
+    order = [i for g in pg for i in g["gates"]]
+    assert order.index(len(spec) - 3) < order.index(len(spec) - 2) < order.index(len(spec) - 1)
+
+
+def test_qasm_reader_matches_oracle(hq):
+    c = hq.Circuit.from_qasm(QASM)
+    n, want = oracle.parse_qasm(QASM)
+    assert c.numQubits == n
+    got = c.gate_array()
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert (a["type"], a["target"], a["control"], a["control2"]) == \
+               (b["type"], b["target"], b["control"], b["control2"])
+        assert np.allclose(a["mat"], b["mat"], atol=1e-15)
+
+
+@pytest.mark.parametrize("bad", ["qreg q[2];\nmeasure q[0];\n", "qreg q[2];\nrx q[0];\n", "h q[0];\n",
+                                 "qreg q[2];\ncx q[0];\n", "qreg q[2];\nu3(0.1,0.2 q[0];\n", "qreg q[2];\nh q[5];\n"])
+def test_qasm_reader_rejects(hq, bad):
+    with pytest.raises(hq.HQError) as e:
+        hq.Circuit.from_qasm(bad)
+    assert e.value.status == hq.HQ_ERR_PARSE
+
+
+def test_options_validation(hq):
+    c = hq.Circuit(10)
+    for k, v in [("mode", 2), ("precision", 5), ("share_qubits", 20), ("nonsense", 1)]:
+        with pytest.raises(hq.HQError):
+            c.set_option(k, v)
+
+
+def test_no_device_error_path(hq):
+    if hq.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(hq.HQError) as e:
+        hq.State(10)
+    assert e.value.status == hq.HQ_ERR_NO_DEVICE
+    c = hq.Circuit(10).add_random_layers(2, 0)
+    with pytest.raises(hq.HQError):
+        c.compile()

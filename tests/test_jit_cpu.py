@@ -1,0 +1,43 @@
+"""CPU check of the specialised OShareMem kernel generator: the HIP source the
+library emits for a group is compiled with g++ and executed with one host
+thread per GPU thread (tests/jit_harness.cpp), then compared with the FP64
+oracle.  No GPU involved."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import group_for_mask as _group_for_mask, rand_state as _rand_state, rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+@pytest.mark.parametrize("n,k,seed,prec,rb", [(10, 9, 31, 0, 0), (11, 10, 32, 0, 3), (12, 11, 33, 1, 0),
+                                              (11, 11, 34, 0, 5)])
+def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb):
+    rs = np.random.default_rng(seed)
+    rest = rs.choice(np.arange(3, n), k - 3, replace=False)
+    mask = 0
+    for q in [0, 1, 2] + [int(x) for x in rest]:
+        mask |= 1 << q
+    g = _group_for_mask(n, mask, 90, seed)
+    src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, g, mask, prec, rb)
+    assert tile == k and (rb == 0 or rbits == rb)
+    ksrc = tmp_path / "kernel.hip"
+    ksrc.write_text(src)
+    exe = tmp_path / "harness"
+    subprocess.run(["g++", "-O1", "-std=c++20", "-pthread", f'-DGROUP_SRC="{ksrc}"',
+                    os.path.join(HERE, "jit_harness.cpp"), "-o", str(exe)], check=True)
+    psi0 = _rand_state(n, seed)
+    dt = np.complex128 if prec == 0 else np.complex64
+    psi0.astype(dt).tofile(tmp_path / "in.bin")
+    np.asarray(coef, dtype=np.float64).tofile(tmp_path / "coef.bin")
+    subprocess.run([str(exe), str(n), str(k), str(1 << tbits), str(tmp_path / "in.bin"),
+                    str(tmp_path / "coef.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
+    got = np.fromfile(tmp_path / "out.bin", dtype=dt)
+    want = oracle.run_plain(g, n, psi0)
+    assert rel_err(got, want) <= (1e-13 if prec == 0 else 2e-6)
