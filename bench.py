@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--reg-qubits", type=int, default=0, help="register qubits per thread (0: default)")
     ap.add_argument("--coalesce", type=int, default=0, help="low qubits always in the tile (0: default)")
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
+    ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
+                    help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-pass", action="store_true",
                     help="before timing, run one zero-gate OShareMem pass (a copy of known size, "
@@ -113,6 +115,8 @@ def main():
         c.set_option("reg_qubits", a.reg_qubits)
     if a.coalesce:
         c.set_option("coalesce", a.coalesce)
+    c.set_option("jit", a.jit)
+    jit_on = a.jit == 1 or (a.jit == -1 and n >= 22)
     t0 = time.perf_counter()
     c.compile()
     compile_s = time.perf_counter() - t0
@@ -151,7 +155,7 @@ def main():
         traffic = _traffic_from_profiles(n)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_sharemem", "avg_launch_ms": round(avg_ms, 4),
+                "kernel": "hq_group" if jit_on else "k_sharemem", "avg_launch_ms": round(avg_ms, 4),
                 "launches_per_step": len(shm) // a.steps}
     kinds = {}
     for k, ms in launches:
