@@ -10,9 +10,10 @@ resident on the device before timing starts.  Prints ONE JSON line.
                     [--layers 30] [--seed 0] [--cpu-gates 45] [--no-cpu]
 
 roofline: the dominant kernel is the OShareMem group kernel; its algorithmic
-traffic per launch is one read + one write of the state, 2 * 2^n * 16 bytes,
-divided by its average launch duration (HIP events recorded on the state's
-stream around every launch in the timed steps).
+traffic per launch is one read + one write of the state, 2 * 2^n * 16 bytes
+(the first group of a run builds |0...0> in registers and only writes), averaged
+over the launches of a step and divided by the average launch duration (HIP
+events recorded on the state's stream around every launch in the timed steps).
 
 cpu_baseline: the FP64 OpenMP per-gate CPU path (oracle/, kernelSimple.cu
 semantics) timed on the first --cpu-gates gates of the same 30-qubit
@@ -148,10 +149,14 @@ def main():
 
     shm = [ms for k, ms in launches if k == hq.KIND_SHAREMEM]
     bytes_per_pass = 2.0 * (1 << n) * amp_bytes
+    # the first group of a run builds |0...0> in registers: it only writes the state
+    init_fused = a.mode != "serial" and bool(sched) and sched[0]["backend"] == hq.HQ_BACKEND_PERGATE and n >= 7
     roof = None
     if shm:
         avg_ms = sum(shm) / len(shm)
-        achieved = bytes_per_pass / (avg_ms * 1e-3) / 1e9
+        per_step = len(shm) // a.steps
+        alg_bytes = bytes_per_pass * (per_step - (0.5 if init_fused else 0.0)) / per_step
+        achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
         traffic = _traffic_from_profiles(n)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -163,7 +168,7 @@ def main():
     groups_pg = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE)
     groups_blas = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_BLAS)
     # whole-step roofline: every group needs one HBM pass (TransMM two)
-    t_ideal = (groups_pg + 2 * groups_blas) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
+    t_ideal = (groups_pg + 2 * groups_blas - (0.5 if init_fused else 0.0)) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
     if not a.no_cpu:

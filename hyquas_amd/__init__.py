@@ -15,6 +15,7 @@ from __future__ import annotations
 import ctypes
 import enum
 import os
+import weakref
 from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint64, c_void_p
 
 import numpy as np
@@ -28,6 +29,8 @@ HQ_C128, HQ_C64 = 0, 1
 HQ_BACKEND_NONE, HQ_BACKEND_PERGATE, HQ_BACKEND_BLAS = 0, 1, 2
 HQ_MODE_SERIAL, HQ_MODE_GROUP, HQ_MODE_BLAS, HQ_MODE_MIX = 0, 1, 3, 4
 KIND_PLAIN, KIND_SHAREMEM, KIND_TRANSPOSE, KIND_DENSE = 0, 1, 2, 3
+HQ_OUT_EXPECT, HQ_OUT_TOPK = 1, 2
+HQ_JOB_QUEUED, HQ_JOB_COMPILED, HQ_JOB_ADMITTED, HQ_JOB_DONE, HQ_JOB_FAILED = range(5)
 
 
 class GateType(enum.IntEnum):
@@ -121,6 +124,28 @@ def _declare(L):
         "hq_circuit_get_gates": (S, [c_void_p, POINTER(HQGate), c_int, POINTER(c_int)]),
         "hq_circuit_add_random_layers": (S, [c_void_p, c_int, c_uint64]),
         "hq_circuit_add_hea_layers": (S, [c_void_p, c_int, c_uint64]),
+        "hq_engine_create": (S, [c_int, c_uint64, POINTER(c_void_p)]),
+        "hq_engine_destroy": (S, [c_void_p]),
+        "hq_engine_set_option": (S, [c_void_p, c_char_p, c_int64]),
+        "hq_engine_set_tile": (S, [c_void_p, c_int, c_int]),
+        "hq_engine_get_tile": (S, [c_void_p, c_int, POINTER(c_int)]),
+        "hq_engine_submit": (S, [c_void_p, c_void_p, c_int, POINTER(c_int64)]),
+        "hq_engine_step": (S, [c_void_p, POINTER(c_int)]),
+        "hq_engine_drain": (S, [c_void_p]),
+        "hq_engine_poll": (S, [c_void_p, POINTER(c_int64), c_int, POINTER(c_int)]),
+        "hq_engine_job_status": (S, [c_void_p, c_int64, POINTER(c_int)]),
+        "hq_engine_result": (S, [c_void_p, c_int64, POINTER(c_double), POINTER(c_uint64), POINTER(c_double),
+                                 POINTER(c_double)]),
+        "hq_engine_release": (S, [c_void_p, c_int64]),
+        "hq_engine_stats": (S, [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64),
+                                POINTER(c_int64)]),
+        "hq_engine_now": (S, [c_void_p, POINTER(c_double)]),
+        "hq_batch_create": (S, [c_void_p, POINTER(c_void_p), c_int, c_int, POINTER(c_void_p)]),
+        "hq_batch_run": (S, [c_void_p, POINTER(c_double)]),
+        "hq_batch_results": (S, [c_void_p, POINTER(c_double), POINTER(c_uint64), POINTER(c_double)]),
+        "hq_batch_info": (S, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_uint64)]),
+        "hq_batch_destroy": (S, [c_void_p]),
+        "hq_state_topk": (S, [c_void_p, c_int, POINTER(c_uint64), POINTER(c_double)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -416,6 +441,13 @@ class State:
         _check(lib().hq_state_inner(self.handle, other.handle, byref(re), byref(im)))
         return complex(re.value, im.value)
 
+    def topk(self, k: int = 16):
+        """The k most probable physical basis states: (indices, probabilities)."""
+        idx = (c_uint64 * k)()
+        prob = (c_double * k)()
+        _check(lib().hq_state_topk(self.handle, k, idx, prob))
+        return list(idx[:]), list(prob[:])
+
     def profile(self, enable: bool):
         _check(lib().hq_profile_enable(self.handle, 1 if enable else 0))
 
@@ -584,3 +616,147 @@ class Circuit:
         _check(lib().hq_circuit_launch_times(self.handle, kinds, ms, cap, byref(cnt)))
         k = min(cnt.value, cap)
         return list(zip(kinds[:k], ms[:k]))
+
+
+# ----------------------------------------------------------------------------- many circuits
+
+class Engine:
+    """Batching / serving engine (hq_engine_*): circuits are submitted, admitted
+    while their device memory fits the budget, and advanced one group per
+    round; ``step()`` launches a round, ``poll()`` returns finished jobs."""
+
+    def __init__(self, precision: int = HQ_C128, hbm_budget: int = 1 << 34):
+        h = c_void_p()
+        _check(lib().hq_engine_create(precision, hbm_budget, byref(h)))
+        self.handle = h
+        self.precision = precision
+        self._batches = weakref.WeakSet()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            for b in list(self._batches):
+                b.close()
+            lib().hq_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, key: str, value: int):
+        _check(lib().hq_engine_set_option(self.handle, key.encode(), int(value)))
+        return self
+
+    def set_tile(self, num_qubits: int, tile: int):
+        _check(lib().hq_engine_set_tile(self.handle, num_qubits, tile))
+        return self
+
+    def tile(self, num_qubits: int) -> int:
+        t = c_int()
+        _check(lib().hq_engine_get_tile(self.handle, num_qubits, byref(t)))
+        return t.value
+
+    def submit(self, circuit: "Circuit", outputs: int = HQ_OUT_EXPECT) -> int:
+        j = c_int64()
+        _check(lib().hq_engine_submit(self.handle, circuit.handle, outputs, byref(j)))
+        return j.value
+
+    def step(self) -> int:
+        p = c_int()
+        _check(lib().hq_engine_step(self.handle, byref(p)))
+        return p.value
+
+    def drain(self):
+        _check(lib().hq_engine_drain(self.handle))
+
+    def poll(self, cap: int = 4096):
+        jobs = (c_int64 * cap)()
+        cnt = c_int()
+        _check(lib().hq_engine_poll(self.handle, jobs, cap, byref(cnt)))
+        return list(jobs[: cnt.value])
+
+    def status(self, job: int) -> int:
+        st = c_int()
+        _check(lib().hq_engine_job_status(self.handle, job, byref(st)))
+        return st.value
+
+    def result(self, job: int, topk: bool = False):
+        """dict(norm, z, zz, times=(submitted, admitted, done)[, topk=(idx, prob)])."""
+        e3 = (c_double * 3)()
+        t3 = (c_double * 3)()
+        idx = (c_uint64 * 16)() if topk else None
+        prob = (c_double * 16)() if topk else None
+        _check(lib().hq_engine_result(self.handle, job, e3, idx, prob, t3))
+        r = dict(norm=e3[0], z=e3[1], zz=e3[2], times=tuple(t3))
+        if topk:
+            r["topk"] = (list(idx[:]), list(prob[:]))
+        return r
+
+    def release(self, job: int):
+        _check(lib().hq_engine_release(self.handle, job))
+
+    def stats(self):
+        u, p, r = c_uint64(), c_uint64(), c_uint64()
+        n = c_int64()
+        _check(lib().hq_engine_stats(self.handle, byref(u), byref(p), byref(r), byref(n)))
+        return dict(bytes_used=u.value, bytes_peak=p.value, bytes_reserved=r.value, rounds=n.value)
+
+    def now(self) -> float:
+        t = c_double()
+        _check(lib().hq_engine_now(self.handle, byref(t)))
+        return t.value
+
+    def batch(self, circuits, outputs: int = HQ_OUT_EXPECT) -> "Batch":
+        return Batch(self, circuits, outputs)
+
+
+class Batch:
+    """A fixed set of circuits captured as one hipGraph (hq_batch_*)."""
+
+    def __init__(self, engine: Engine, circuits, outputs: int = HQ_OUT_EXPECT):
+        self.engine = engine
+        self.circuits = list(circuits)
+        arr = (c_void_p * len(self.circuits))(*[c.handle.value for c in self.circuits])
+        h = c_void_p()
+        _check(lib().hq_batch_create(engine.handle, arr, len(self.circuits), outputs, byref(h)))
+        self.handle = h
+        self.outputs = outputs
+        engine._batches.add(self)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().hq_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self) -> float:
+        ms = c_double()
+        _check(lib().hq_batch_run(self.handle, byref(ms)))
+        return ms.value
+
+    def info(self):
+        c, r, l = c_int(), c_int(), c_int()
+        b = c_uint64()
+        _check(lib().hq_batch_info(self.handle, byref(c), byref(r), byref(l), byref(b)))
+        return dict(count=c.value, rounds=r.value, launches=l.value, device_bytes=b.value)
+
+    def results(self):
+        """(expect[count, 3] = norm, sum Z, sum ZZ; topk (idx[count, 16], prob[count, 16]) or None)."""
+        cnt = len(self.circuits)
+        e = np.zeros((cnt, 3))
+        if self.outputs & HQ_OUT_TOPK:
+            idx = np.zeros((cnt, 16), dtype=np.uint64)
+            prob = np.zeros((cnt, 16))
+            _check(lib().hq_batch_results(self.handle, e.ctypes.data_as(POINTER(c_double)),
+                                          idx.ctypes.data_as(POINTER(c_uint64)),
+                                          prob.ctypes.data_as(POINTER(c_double))))
+            return e, (idx, prob)
+        _check(lib().hq_batch_results(self.handle, e.ctypes.data_as(POINTER(c_double)), None, None))
+        return e, None

@@ -120,6 +120,7 @@ struct ShmHeader {
 // Host-side result of lowering + register-stage planning of one group.
 struct ShmPlan {
     int n = 0, precision = 0, k = 0, rb = 0, tb = 0;
+    bool init = false;        // start from |0...0> instead of loading (first group of a run)
     uint64_t shared_mask = 0;
     ShmHeader hdr;
     std::vector<ShmStage> stages;

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <mutex>
@@ -69,6 +70,8 @@ struct Emitter {
     std::ostringstream o;
     int NR, NT, K;
     bool f32;
+    bool nt_ld, nt_st;       // non-temporal HBM loads / stores
+    int bmap;                // block index -> tile: 0 natural, 1 reversed bits, 2 contiguous per XCD
     std::vector<int> name;   // register j is held in variable r<name[j]>
 
     Emitter(const ShmPlan& p, std::vector<double>* c) : P(p), coef(c)
@@ -77,6 +80,15 @@ struct Emitter {
         NT = 1 << p.tb;
         K = p.k;
         f32 = p.precision != HQ_C128;
+        // non-temporal accesses measured slower for these kernels on MI355X
+        // (7.95 vs 7.17 ms per 30-qubit pass); HQ_JIT_NT=1 turns them on
+        const char* env = getenv("HQ_JIT_NT");
+        const int ntm = env ? atoi(env) : 0;   // 1 both, 2 loads, 3 stores
+        nt_ld = ntm == 1 || ntm == 2;
+        nt_st = ntm == 1 || ntm == 3;
+        const char* bm = getenv("HQ_JIT_BMAP");
+        bmap = bm ? atoi(bm) : 0;
+        if ((bmap == 2 && P.hdr.num_block_bits < 3) || (bmap == 1 && P.hdr.num_block_bits == 0)) bmap = 0;
         name.resize(NR);
         for (int j = 0; j < NR; j++) name[j] = j;
     }
@@ -239,10 +251,23 @@ struct Emitter {
         const int nst = (int)P.stages.size();
         o << "typedef " << (f32 ? "float" : "double") << " R;\n";
         o << "struct __attribute__((aligned(" << (f32 ? 8 : 16) << "))) C { R x, y; };\n";
+        if (nt_ld || nt_st) o << "typedef R V2 __attribute__((ext_vector_type(2)));\n";
+        if (nt_ld)
+            o << "__device__ __forceinline__ C ld(const C* p) { const V2 v = __builtin_nontemporal_load((const V2*)p); "
+                 "C c; c.x = v.x; c.y = v.y; return c; }\n";
+        if (nt_st)
+            o << "__device__ __forceinline__ void st(C* p, const C c) { V2 v; v.x = c.x; v.y = c.y; "
+                 "__builtin_nontemporal_store(v, (V2*)p); }\n";
         o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") void hq_group(C* __restrict__ psi, const R* __restrict__ cf)\n{\n";
         if (nst > 1) o << "    __shared__ C lds[" << (1 << K) << "];\n";
         o << "    const unsigned t = threadIdx.x;\n";
-        o << "    const unsigned long long b = blockIdx.x;\n";
+        if (bmap == 2)
+            o << "    const unsigned long long b = (unsigned long long)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);\n";
+        else if (bmap == 1)
+            o << "    const unsigned long long b = __builtin_bitreverse64((unsigned long long)blockIdx.x) >> "
+              << (64 - (int)P.hdr.num_block_bits) << ";\n";
+        else
+            o << "    const unsigned long long b = blockIdx.x;\n";
         o << "    unsigned long long base = 0;\n";
         for (uint32_t j = 0; j < P.hdr.num_block_bits; j++) {
             int p = ctz64(P.hdr.block_phys[j]);
@@ -258,14 +283,25 @@ struct Emitter {
             const ShmStage& st = P.stages[0];
             o << "    {\n";
             lane_phys(st, "lp");
-            o << "    const C* __restrict__ src = psi + (base + lp);\n";
-            for (int j = 0; j < NR; j++) o << "    r" << j << " = src[" << reg_off(st, j) << "ull];\n";
+            if (P.init) {
+                // first group of a run: the tile of |0...0> is built in registers
+                o << "    r0.x = (base + lp) == 0 ? R(1) : R(0); r0.y = R(0);\n";
+                for (int j = 1; j < NR; j++) o << "    r" << j << ".x = R(0); r" << j << ".y = R(0);\n";
+            } else {
+                o << "    const C* __restrict__ src = psi + (base + lp);\n";
+                for (int j = 0; j < NR; j++) {
+                    if (nt_ld) o << "    r" << j << " = ld(src + " << reg_off(st, j) << "ull);\n";
+                    else o << "    r" << j << " = src[" << reg_off(st, j) << "ull];\n";
+                }
+            }
             o << "    }\n";
         }
         for (int s = 0; s < nst; s++) {
             const ShmStage& st = P.stages[s];
             if (s > 0) {
                 const ShmStage& prev = P.stages[s - 1];
+                // every lane has read the previous exchange before it is overwritten
+                if (s > 1) o << "    __syncthreads();\n";
                 o << "    {\n";
                 lane_lds(prev, "sw");
                 for (int j = 0; j < NR; j++)
@@ -284,7 +320,10 @@ struct Emitter {
             o << "    {\n";
             lane_phys(st, "lq");
             o << "    C* __restrict__ dst = psi + (base + lq);\n";
-            for (int j = 0; j < NR; j++) o << "    dst[" << reg_off(st, j) << "ull] = " << r(j) << ";\n";
+            for (int j = 0; j < NR; j++) {
+                if (nt_st) o << "    st(dst + " << reg_off(st, j) << "ull, " << r(j) << ");\n";
+                else o << "    dst[" << reg_off(st, j) << "ull] = " << r(j) << ";\n";
+            }
             o << "    }\n";
         }
         o << "}\n";

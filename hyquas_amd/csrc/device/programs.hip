@@ -219,6 +219,7 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
             out->assign(reqs.size(), nullptr);
             return st;
         }
+        (*out)[i]->plan->init = reqs[i].init;
     }
     if (!jit || reqs.empty()) return HQ_OK;
     std::vector<const ShmPlan*> plans;

@@ -19,194 +19,39 @@
 // the host picks lane bits 0..2 so that 8-lane groups hit distinct 16-byte
 // bank groups.
 
-#include "hq_ops.h"
+#include "sharemem_dev.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace hq {
 
 // ---- device side ----------------------------------------------------------------------
 
-template <typename R, int RB, int CLS, int P>
-__device__ __forceinline__ void reg_pairs(cplx<R> (&a)[1 << RB], uint32_t creg, const R* m)
+template <typename R, int RB, int TB, bool NT>
+__global__ __launch_bounds__(1 << TB) void k_sharemem(cplx<R>* __restrict__ psi, const char* __restrict__ prog,
+                                                     uint32_t mode)
 {
-#pragma unroll
-    for (int j = 0; j < (1 << RB); j++) {
-        if (j & (1 << P)) continue;
-        if ((j & creg) != creg) continue;
-        pair_update<CLS, R>(a[j], a[j | (1 << P)], m);
-    }
-}
-
-template <typename R, int RB, int P>
-__device__ __forceinline__ void reg_op(cplx<R> (&a)[1 << RB], int cls, uint32_t creg, const R* m)
-{
-    switch (cls) {
-    case SHM_SWAP: reg_pairs<R, RB, SHM_SWAP, P>(a, creg, m); break;
-    case SHM_Y: reg_pairs<R, RB, SHM_Y, P>(a, creg, m); break;
-    case SHM_REAL: reg_pairs<R, RB, SHM_REAL, P>(a, creg, m); break;
-    case SHM_RX: reg_pairs<R, RB, SHM_RX, P>(a, creg, m); break;
-    case SHM_GEN: reg_pairs<R, RB, SHM_GEN, P>(a, creg, m); break;
-    case SHM_DIAG: reg_pairs<R, RB, SHM_DIAG, P>(a, creg, m); break;
-    default: reg_pairs<R, RB, SHM_PHASE, P>(a, creg, m); break;
-    }
-}
-
-template <typename R, int RB>
-__device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const ShmOp* __restrict__ op, uint32_t tid,
-                                         uint64_t blk)
-{
-    const uint64_t cblk = op->cblk;
-    if ((blk & cblk) != cblk) return;                 // block control not set (uniform)
-    const uint32_t cthr = op->cthr;
-    if ((tid & cthr) != cthr) return;                 // thread control not set (per lane)
-    R m[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) m[i] = (R)op->m[i];
-    const uint32_t creg = op->creg;
-    const int cls = op->cls;
-    const int tpos = op->tpos;
-    if (op->where == SHM_IN_REG) {
-        switch (tpos) {
-        case 0: reg_op<R, RB, 0>(a, cls, creg, m); break;
-        case 1: if constexpr (RB > 1) reg_op<R, RB, 1>(a, cls, creg, m); break;
-        case 2: if constexpr (RB > 2) reg_op<R, RB, 2>(a, cls, creg, m); break;
-        case 3: if constexpr (RB > 3) reg_op<R, RB, 3>(a, cls, creg, m); break;
-        default: if constexpr (RB > 4) reg_op<R, RB, 4>(a, cls, creg, m); break;
-        }
-        return;
-    }
-    // diagonal gate whose target is a thread bit or a block bit: one factor
-    // per lane for every register amplitude
-    uint32_t hb = op->where == SHM_IN_THREAD ? (tid >> tpos) & 1u : (uint32_t)((blk >> tpos) & 1u);
-    if (cls == SHM_PHASE && !hb) return;
-    const R fr = hb ? m[6] : m[0];
-    const R fi = hb ? m[7] : m[1];
-#pragma unroll
-    for (int j = 0; j < (1 << RB); j++) {
-        if ((j & creg) != creg) continue;
-        scale(a[j], fr, fi);
-    }
-}
-
-template <int K>
-__device__ __forceinline__ uint32_t swz(uint32_t x, const uint32_t* __restrict__ col)
-{
-    uint32_t f = 0;
-#pragma unroll
-    for (int i = 3; i < K; i++) f ^= ((x >> i) & 1u) ? col[i] : 0u;
-    return x ^ f;
-}
-
-template <typename R, int RB, int TB>
-__global__ __launch_bounds__(1 << TB) void k_sharemem(cplx<R>* __restrict__ psi, const char* __restrict__ prog)
-{
-    constexpr int NR = 1 << RB;
-    constexpr int K = RB + TB;
-    __shared__ cplx<R> lds[1 << K];
-
-    const ShmHeader* __restrict__ H = (const ShmHeader*)prog;
-    const ShmStage* __restrict__ ST = (const ShmStage*)(prog + sizeof(ShmHeader));
-    const uint32_t nst = H->num_stages;
-    const ShmOp* __restrict__ OP = (const ShmOp*)(prog + sizeof(ShmHeader) + nst * sizeof(ShmStage));
-    const uint32_t tid = threadIdx.x;
-    const uint64_t blk = blockIdx.x;
-
-    // block base address: pdep(blk, block bits)
-    uint64_t base = 0;
-    const uint32_t nbb = H->num_block_bits;
-    for (uint32_t j = 0; j < nbb; j++)
-        if ((blk >> j) & 1) base |= H->block_phys[j];
-
-    uint32_t col[K];
-#pragma unroll
-    for (int i = 0; i < K; i++) col[i] = H->swz_col[i];
-
-    cplx<R> a[NR];
-
-    // mapping of stage 0
-    uint32_t rbit[RB];
-    uint64_t rphys[RB];
-#pragma unroll
-    for (int i = 0; i < RB; i++) {
-        rbit[i] = 1u << ST[0].reg_bits[i];
-        rphys[i] = H->tile_phys[ST[0].reg_bits[i]];
-    }
-    uint32_t et = 0;
-    uint64_t pt = 0;
-#pragma unroll
-    for (int i = 0; i < TB; i++) {
-        uint32_t b = ST[0].thr_bits[i];
-        if ((tid >> i) & 1) { et |= 1u << b; pt |= H->tile_phys[b]; }
-    }
-    const cplx<R>* __restrict__ src = psi + (base | pt);
-#pragma unroll
-    for (int j = 0; j < NR; j++) {
-        uint64_t off = 0;
-#pragma unroll
-        for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
-        a[j] = src[off];
-    }
-
-    for (uint32_t s = 0; s < nst; s++) {
-        if (s > 0) {
-            // exchange: write under the old mapping, read under the new one
-            const uint32_t st = swz<K>(et, col);
-#pragma unroll
-            for (int j = 0; j < NR; j++) {
-                uint32_t off = 0;
-#pragma unroll
-                for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rbit[i];
-                lds[st ^ swz<K>(off, col)] = a[j];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < RB; i++) rbit[i] = 1u << ST[s].reg_bits[i];
-            et = 0;
-#pragma unroll
-            for (int i = 0; i < TB; i++)
-                if ((tid >> i) & 1) et |= 1u << ST[s].thr_bits[i];
-            const uint32_t st2 = swz<K>(et, col);
-#pragma unroll
-            for (int j = 0; j < NR; j++) {
-                uint32_t off = 0;
-#pragma unroll
-                for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rbit[i];
-                a[j] = lds[st2 ^ swz<K>(off, col)];
-            }
-        }
-        const uint32_t ob = ST[s].op_begin, oe = ST[s].op_end;
-        for (uint32_t o = ob; o < oe; o++) apply_op<R, RB>(a, OP + o, tid, blk);
-    }
-
-    // store under the mapping of the last stage
-    pt = 0;
-#pragma unroll
-    for (int i = 0; i < TB; i++)
-        if ((et >> ST[nst - 1].thr_bits[i]) & 1) pt |= H->tile_phys[ST[nst - 1].thr_bits[i]];
-#pragma unroll
-    for (int i = 0; i < RB; i++) rphys[i] = H->tile_phys[ST[nst - 1].reg_bits[i]];
-    cplx<R>* __restrict__ dst = psi + (base | pt);
-#pragma unroll
-    for (int j = 0; j < NR; j++) {
-        uint64_t off = 0;
-#pragma unroll
-        for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
-        dst[off] = a[j];
-    }
+    __shared__ cplx<R> lds[1 << (RB + TB)];
+    tile_pass<R, RB, TB, NT>(psi, prog, blockIdx.x, lds, mode, 0, nullptr);
 }
 
 // ---- launch table -------------------------------------------------------------------------
-
-using ShmKernel = void (*)(void*, const char*);
 
 template <typename R, int RB, int TB>
 static hq_status launch_variant(hq_state s, hq_program p)
 {
     uint64_t blocks = (uint64_t)1 << (p->n - p->tile);
-    hipLaunchKernelGGL((k_sharemem<R, RB, TB>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
-                       (cplx<R>*)s->data, (const char*)p->d_prog);
+    const uint32_t mode = p->plan && p->plan->init ? TILE_INIT : 0u;
+    // non-temporal accesses (HQ_SHM_NT=1) stream the state past the caches
+    static const bool nt = getenv("HQ_SHM_NT") && atoi(getenv("HQ_SHM_NT")) != 0;
+    if (nt)
+        hipLaunchKernelGGL((k_sharemem<R, RB, TB, true>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
+                           (cplx<R>*)s->data, (const char*)p->d_prog, mode);
+    else
+        hipLaunchKernelGGL((k_sharemem<R, RB, TB, false>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
+                           (cplx<R>*)s->data, (const char*)p->d_prog, mode);
     HQ_HIP(hipGetLastError());
     return HQ_OK;
 }

@@ -14,10 +14,6 @@ void add_random_layers(Circuit& c, int layers, uint64_t seed);
 void add_hea_layers(Circuit& c, int layers, uint64_t seed);
 }  // namespace hq
 
-struct hq_circuit_s {
-    std::unique_ptr<Circuit> c;
-};
-
 #define HQ_CARG(cond, msg)                              \
     do {                                                \
         if (!(cond)) { hq::set_error(msg); return HQ_ERR_ARG; } \

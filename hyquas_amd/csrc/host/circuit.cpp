@@ -128,8 +128,11 @@ hq_status Circuit::runChecked(bool copy_back, bool destroy, long long* us)
     }
     rc = ensureState();
     if (rc != HQ_OK) return rc;
-    rc = hq_state_reset(state);
-    if (rc != HQ_OK) return rc;
+    // a fused first group writes |0...0> evolved by its gates over the whole state
+    if (params.mode == HQ_MODE_SERIAL || !schedule.initFused) {
+        rc = hq_state_reset(state);
+        if (rc != HQ_OK) return rc;
+    }
     hq_profile_enable(state, profileEvents ? 1 : 0);
     rc = hq_state_sync(state);
     if (rc != HQ_OK) return rc;

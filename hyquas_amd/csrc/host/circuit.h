@@ -1,6 +1,7 @@
 // circuit.h -- public Circuit API (src/circuit.h:9-46), single MI355X GPU.
 #pragma once
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -89,4 +90,9 @@ public:
     static void add(const char* format, ...);
     static void print();
     static void enable(bool on);
+};
+
+// Opaque handle of the C ABI (hq_circuit): owns one Circuit.
+struct hq_circuit_s {
+    std::unique_ptr<Circuit> c;
 };

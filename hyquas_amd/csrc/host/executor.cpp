@@ -32,8 +32,12 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
     State st(n);
     std::vector<hq::SharememRequest> reqs;
     std::vector<GateGroup*> owners;
+    bool first = true;
+    sch.initFused = false;
     for (auto& lg : sch.localGroups) {
         for (auto& gg : lg.fullGroups) {
+            const bool isFirst = first;
+            first = false;
             if (gg.program) { hq_program_destroy(gg.program); gg.program = nullptr; }
             if (gg.backend == Backend::PerGate) {
                 const int k = std::min(p.localSize, n);
@@ -56,6 +60,9 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                         h.control2 = g.controlQubit2 >= 0 ? st.pos[g.controlQubit2] : -1;
                         r.gates.push_back(h);
                     }
+                    // the run starts from |0...0>: the first pass builds it in registers
+                    r.init = isFirst;
+                    sch.initFused = sch.initFused || isFirst;
                     reqs.push_back(std::move(r));
                     owners.push_back(&gg);
                 }

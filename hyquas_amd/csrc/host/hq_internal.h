@@ -19,6 +19,7 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
 struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;
+    bool init = false;   // first group of a run: build |0...0> instead of loading the state
 };
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
                                   std::vector<hq_program>* out);

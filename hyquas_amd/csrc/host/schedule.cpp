@@ -157,6 +157,7 @@ void Schedule::clear()
 {
     localGroups.clear();
     finalState = State();
+    initFused = false;
 }
 
 void Schedule::dump(int numQubits) const

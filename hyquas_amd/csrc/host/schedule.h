@@ -59,6 +59,7 @@ struct LocalGroup {
 struct Schedule {
     std::vector<LocalGroup> localGroups;   // one on a single GPU
     State finalState;
+    bool initFused = false;                // the first group's program builds |0...0> itself
     void dump(int numQubits) const;        // SHOW_SCHEDULE print (src/schedule.cpp:66-154)
     void clear();
 };

@@ -76,6 +76,8 @@ typedef struct hq_gate {
 typedef struct hq_state_s   *hq_state;
 typedef struct hq_program_s *hq_program;
 typedef struct hq_circuit_s *hq_circuit;
+typedef struct hq_engine_s  *hq_engine;
+typedef struct hq_batch_s   *hq_batch;
 
 /* ======================================================================== */
 /* Library                                                                  */
@@ -311,6 +313,66 @@ hq_status hq_circuit_get_gates(hq_circuit c, hq_gate *out, int cap, int *count);
  * Append to c. */
 hq_status hq_circuit_add_random_layers(hq_circuit c, int layers, uint64_t seed);
 hq_status hq_circuit_add_hea_layers(hq_circuit c, int layers, uint64_t seed);
+
+/* ---- many circuits: batching and serving -------------------------------------
+ *
+ * No counterpart in the reference, which runs one circuit per process
+ * (src/main.cpp:233-251).  An engine compiles circuits on host threads
+ * (OShareMem groups only; circuits of at most "whole_max" qubits are one group
+ * whose tile is the whole state), admits them in submission order while their
+ * device memory fits `hbm_budget` bytes, and advances every admitted circuit by
+ * one group per round (hq_engine_step), all circuits of one tile shape in one
+ * launch.  Circuits submitted while others run join at the next round.
+ * Readouts per circuit: {norm, sum_q <Z_q>, sum_{q<n-1} <Z_q Z_{q+1}>} over
+ * logical qubits (always) and, with HQ_OUT_TOPK, the 16 most probable basis
+ * states (probability descending, ties by smaller index).  Reductions run in a
+ * fixed order, so a result does not depend on the other circuits. */
+enum { HQ_OUT_EXPECT = 1, HQ_OUT_TOPK = 2 };
+enum { HQ_JOB_QUEUED = 0, HQ_JOB_COMPILED = 1, HQ_JOB_ADMITTED = 2, HQ_JOB_DONE = 3, HQ_JOB_FAILED = 4 };
+
+hq_status hq_engine_create(int precision, uint64_t hbm_budget, hq_engine *out);
+hq_status hq_engine_destroy(hq_engine e);
+/* "whole_max" (13), "share_qubits" (12), "coalesce" (4), "compile_threads" (4,
+ * before the first submit), "max_inflight" (2 rounds queued ahead of the host). */
+hq_status hq_engine_set_option(hq_engine e, const char *key, int64_t value);
+/* Tile qubits for circuits of num_qubits qubits (0: the whole_max/share_qubits
+ * rule); hq_engine_get_tile reports the tile in effect. */
+hq_status hq_engine_set_tile(hq_engine e, int num_qubits, int tile);
+hq_status hq_engine_get_tile(hq_engine e, int num_qubits, int *tile);
+/* Queue a copy of c's gates (7..34 qubits); outputs = HQ_OUT_* bits. */
+hq_status hq_engine_submit(hq_engine e, hq_circuit c, int outputs, int64_t *job);
+/* Retire finished rounds, admit what fits, launch one round.  Returns without
+ * waiting for the round (at most max_inflight rounds are queued). *pending =
+ * circuits submitted and not yet done. */
+hq_status hq_engine_step(hq_engine e, int *pending);
+/* Step until every submitted circuit is done. */
+hq_status hq_engine_drain(hq_engine e);
+/* Circuits finished (or failed) since the last poll. */
+hq_status hq_engine_poll(hq_engine e, int64_t *jobs, int cap, int *count);
+hq_status hq_engine_job_status(hq_engine e, int64_t job, int *status);
+/* expect3 = {norm, sum Z, sum ZZ}; topk_* [16] (needs HQ_OUT_TOPK); times3 =
+ * {submitted, admitted, done} in seconds on the engine clock.  A failed
+ * circuit returns its error. */
+hq_status hq_engine_result(hq_engine e, int64_t job, double *expect3, uint64_t *topk_idx, double *topk_prob,
+                           double *times3);
+hq_status hq_engine_release(hq_engine e, int64_t job);
+hq_status hq_engine_stats(hq_engine e, uint64_t *bytes_used, uint64_t *bytes_peak, uint64_t *bytes_reserved,
+                          int64_t *rounds);
+hq_status hq_engine_now(hq_engine e, double *seconds);
+
+/* A fixed set of circuits on an idle engine: compiled, resident and captured
+ * as one hipGraph at creation; hq_batch_run replays it (device_ms = its
+ * duration by HIP events) and results are read with hq_batch_results
+ * (expect3[3 * count], topk_*[16 * count]). */
+hq_status hq_batch_create(hq_engine e, const hq_circuit *circuits, int count, int outputs, hq_batch *out);
+hq_status hq_batch_run(hq_batch b, double *device_ms);
+hq_status hq_batch_results(hq_batch b, double *expect3, uint64_t *topk_idx, double *topk_prob);
+hq_status hq_batch_info(hq_batch b, int *count, int *rounds, int *launches, uint64_t *device_bytes);
+hq_status hq_batch_destroy(hq_batch b);
+
+/* The k (<= 16) most probable physical basis states of s, probability
+ * descending, ties by smaller index; FP64.  Synchronous. */
+hq_status hq_state_topk(hq_state s, int k, uint64_t *idx, double *prob);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,178 @@
+"""GPU parity of the batching / serving engine (hq_engine_*, hq_batch_*) and
+the top-k readout against the FP64 oracle, plus the properties the forms
+promise: a result is bit-identical alone and in a mix, graph replays repeat
+bit-for-bit, admission never exceeds the HBM budget, complex64 stays within
+1e-5 of complex128."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import hea_layers, random_layers, to_oracle
+from test_engine_cpu import readouts as _readouts
+
+pytestmark = pytest.mark.gpu
+
+
+def _topk_ref(psi, k=16):
+    p = np.abs(psi) ** 2
+    order = np.lexsort((np.arange(p.size), -p))[:k]
+    return [int(i) for i in order], [float(p[i]) for i in order]
+
+
+def _hea(hq, n, depth, seed):
+    return hq.Circuit(n).add_hea_layers(depth, seed)
+
+
+def _rl(hq, n, depth, seed):
+    return hq.Circuit(n).add_random_layers(depth, seed)
+
+
+@pytest.mark.parametrize("n,depth", [(10, 6), (13, 4), (14, 5), (17, 3)])
+def test_engine_hea_matches_oracle(gpu, n, depth):
+    hq = gpu
+    e = hq.Engine(hq.HQ_C128, 1 << 30)
+    c = _hea(hq, n, depth, seed=n)
+    j = e.submit(c, hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+    e.drain()
+    r = e.result(j, topk=True)
+    psi = oracle.run_plain(to_oracle(hea_layers(n, depth, n)), n)
+    norm, z, zz = _readouts(psi, n)
+    assert abs(r["norm"] - 1) < 1e-12
+    assert abs(r["z"] - z) < 1e-10 * n and abs(r["zz"] - zz) < 1e-10 * n
+    idx, prob = _topk_ref(psi)
+    assert r["topk"][0] == idx
+    np.testing.assert_allclose(r["topk"][1], prob, rtol=1e-10, atol=1e-15)
+    e.close()
+
+
+@pytest.mark.parametrize("n", [12, 16, 20])
+def test_engine_random_layers_matches_oracle(gpu, n):
+    hq = gpu
+    e = hq.Engine(hq.HQ_C128, 1 << 30)
+    depth = 6
+    j = e.submit(_rl(hq, n, depth, seed=5), hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+    e.drain()
+    r = e.result(j, topk=True)
+    psi = oracle.run_plain(to_oracle(random_layers(n, depth, 5)), n)
+    _, z, zz = _readouts(psi, n)
+    assert abs(r["z"] - z) < 1e-10 * n and abs(r["zz"] - zz) < 1e-10 * n
+    idx, prob = _topk_ref(psi)
+    assert r["topk"][0] == idx
+    np.testing.assert_allclose(r["topk"][1], prob, rtol=1e-10, atol=1e-15)
+
+
+def test_alone_equals_in_mix(gpu):
+    hq = gpu
+    probe = [(11, 7), (15, 5), (19, 4)]
+    alone = {}
+    for n, d in probe:
+        e = hq.Engine(hq.HQ_C128, 1 << 30)
+        j = e.submit(_hea(hq, n, d, 100 + n), hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+        e.drain()
+        alone[n] = e.result(j, topk=True)
+        e.close()
+    # the same circuits interleaved with others, submitted while rounds run
+    e = hq.Engine(hq.HQ_C128, 1 << 30)
+    ids = {}
+    others = [(10, 9), (21, 3), (13, 12), (16, 6), (12, 3)]
+    for k, (n, d) in enumerate(others[:2]):
+        e.submit(_rl(hq, n, d, k), hq.HQ_OUT_EXPECT)
+    e.step()
+    for n, d in probe:
+        ids[n] = e.submit(_hea(hq, n, d, 100 + n), hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+        e.step()
+        for k, (m, dd) in enumerate(others[2:]):
+            e.submit(_hea(hq, m, dd, 7 * k + n), hq.HQ_OUT_EXPECT)
+    e.drain()
+    for n, _ in probe:
+        r = e.result(ids[n], topk=True)
+        a = alone[n]
+        assert (r["norm"], r["z"], r["zz"]) == (a["norm"], a["z"], a["zz"])
+        assert r["topk"] == a["topk"]
+
+
+def test_batch_graph_matches_engine_and_replays(gpu):
+    hq = gpu
+    rng = np.random.default_rng(11)
+    specs = [(int(rng.integers(10, 19)), int(rng.integers(2, 9))) for _ in range(24)]
+    circuits = [_hea(hq, n, d, 1000 + i) for i, (n, d) in enumerate(specs)]
+    e = hq.Engine(hq.HQ_C128, 1 << 31)
+    b = e.batch(circuits, hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+    b.run()
+    r1, t1 = b.results()
+    e2 = hq.Engine(hq.HQ_C128, 1 << 31)
+    jobs = [e2.submit(c, hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK) for c in circuits]
+    e2.drain()
+    for i, j in enumerate(jobs):
+        r = e2.result(j, topk=True)
+        assert (r["norm"], r["z"], r["zz"]) == tuple(r1[i])
+        assert r["topk"][0] == [int(x) for x in t1[0][i]]
+    for _ in range(3):
+        b.run()
+        r2, t2 = b.results()
+        assert np.array_equal(r1, r2) and np.array_equal(t1[0], t2[0]) and np.array_equal(t1[1], t2[1])
+    # spot-check two circuits against the oracle
+    for i in (0, len(specs) - 1):
+        n, d = specs[i]
+        psi = oracle.run_plain(to_oracle(hea_layers(n, d, 1000 + i)), n)
+        _, z, zz = _readouts(psi, n)
+        assert abs(r1[i][1] - z) < 1e-10 * n and abs(r1[i][2] - zz) < 1e-10 * n
+    b.close()
+
+
+def test_budget_admission(gpu):
+    hq = gpu
+    budget = 1 << 27  # 128 MiB: about three 20-qubit complex128 states at a time
+    e = hq.Engine(hq.HQ_C128, budget)
+    jobs = [e.submit(_rl(hq, 20, 4, s), hq.HQ_OUT_EXPECT) for s in range(8)]
+    big = e.submit(_rl(hq, 26, 2, 0), hq.HQ_OUT_EXPECT)  # 1 GiB: can never fit
+    e.drain()
+    st = e.stats()
+    assert st["bytes_peak"] <= budget and st["bytes_reserved"] <= budget
+    assert e.status(big) == hq.HQ_JOB_FAILED
+    with pytest.raises(hq.HQError):
+        e.result(big)
+    ref = oracle.run_plain(to_oracle(random_layers(20, 4, 3)), 20)
+    _, z, zz = _readouts(ref, 20)
+    r = e.result(jobs[3])
+    assert abs(r["z"] - z) < 1e-9 and abs(r["zz"] - zz) < 1e-9
+    assert all(e.status(j) == hq.HQ_JOB_DONE for j in jobs)
+    assert st["bytes_used"] == 0
+
+
+def test_c64_close_to_c128(gpu):
+    hq = gpu
+    res = {}
+    for prec in (hq.HQ_C128, hq.HQ_C64):
+        e = hq.Engine(prec, 1 << 30)
+        js = [e.submit(_hea(hq, n, 10, n), hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK) for n in (11, 14, 18)]
+        e.drain()
+        res[prec] = [e.result(j, topk=True) for j in js]
+    for a, b in zip(res[hq.HQ_C128], res[hq.HQ_C64]):
+        assert abs(a["z"] - b["z"]) < 1e-4 and abs(a["zz"] - b["zz"]) < 1e-4
+        assert abs(b["norm"] - 1) < 1e-5
+        np.testing.assert_allclose(a["topk"][1], b["topk"][1], atol=1e-6)
+
+
+def test_state_topk(gpu):
+    hq = gpu
+    for n in (7, 12, 21):
+        s = hq.State(n)
+        psi = np.random.default_rng(n).normal(size=1 << n) + 0j
+        psi /= np.linalg.norm(psi)
+        s.upload(psi)
+        idx, prob = s.topk(16)
+        ridx, rprob = _topk_ref(psi)
+        assert idx == ridx
+        np.testing.assert_allclose(prob, rprob, rtol=1e-14)
+        s.close()
+
+
+def test_state_topk_ties(gpu):
+    hq = gpu
+    s = hq.State(13)
+    psi = np.full(1 << 13, 1 / np.sqrt(1 << 13), dtype=np.complex128)
+    s.upload(psi)
+    idx, prob = s.topk(16)
+    assert idx == list(range(16))
+    s.close()
