@@ -90,6 +90,15 @@ def main():
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
                     help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["sim", "batch", "serve"], default="sim",
+                    help="sim: one 30-qubit circuit (default); batch: 4096 ansatz circuits; "
+                         "serve: Poisson request stream")
+    ap.add_argument("--count", type=int, default=4096, help="batch: circuits")
+    ap.add_argument("--plain-sample", type=int, default=256,
+                    help="batch: circuits of the plain FP64 one-at-a-time baseline (evenly spaced)")
+    ap.add_argument("--no-autotune", action="store_true", help="batch: default tiles instead of autotuned ones")
+    ap.add_argument("--requests", type=int, default=120, help="serve: requests per rate")
+    ap.add_argument("--rates", default="5,20,80", help="serve: arrival rates (requests/s), comma separated")
     ap.add_argument("--copy-pass", action="store_true",
                     help="before timing, run one zero-gate OShareMem pass (a copy of known size, "
                          "2 * 2^n * amplitude bytes) to calibrate HBM counters")
@@ -102,6 +111,11 @@ def main():
         sys.exit(2)
 
     import hyquas_amd as hq
+
+    if a.form == "batch":
+        return main_batch(a, hq)
+    if a.form == "serve":
+        return main_serve(a, hq)
 
     n, layers = a.qubits, a.layers
     prec = hq.HQ_C128 if a.precision == "c128" else hq.HQ_C64
@@ -201,6 +215,139 @@ def main():
         "compile_s": round(compile_s, 3),
         "norm_error": norm_err,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _fidelity(hq, spec, kinds="hea"):
+    """1 - |<psi64|psi128>|^2 for one circuit (OShareMem path, both precisions)."""
+    from hyquas_amd.serving import build_circuit
+    n, d, seed = spec
+    cs = []
+    for prec in (hq.HQ_C128, hq.HQ_C64):
+        c = build_circuit(hq, kinds, n, d, seed)
+        c.set_option("mode", hq.HQ_MODE_GROUP)
+        c.set_option("precision", prec)
+        c.run(copy_back=False, destroy=False)
+        cs.append(c)
+    ov = cs[0].state_view(hq.HQ_C128).inner(cs[1].state_view(hq.HQ_C64))
+    for c in cs:
+        c.close()
+    return 1.0 - abs(ov) ** 2
+
+
+def main_batch(a, hq):
+    """Batching form: `count` HEA circuits, n log-uniform in [10, 22], depth in [4, 40];
+    output sum_i <Z_i Z_{i+1}> per circuit.  Value: circuits/s of the complex64
+    batch (one hipGraph replay of every round), next to complex128 and the
+    plain FP64 one-at-a-time path on the same GPU."""
+    import numpy as np
+    from hyquas_amd.serving import autotune_tiles, build_circuit, hea_batch
+    specs = hea_batch(a.count, a.seed)
+    circuits = [build_circuit(hq, "hea", n, d, s) for n, d, s in specs]
+    res = {}
+    zz = {}
+    for name, prec in (("c64", hq.HQ_C64), ("c128", hq.HQ_C128)):
+        e = hq.Engine(prec, 280 * 10 ** 9)
+        t0 = time.perf_counter()
+        tiles = {} if a.no_autotune else autotune_tiles(hq, e, specs, circuits)
+        tune_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b = e.batch(circuits, hq.HQ_OUT_EXPECT)
+        build_s = time.perf_counter() - t0
+        for _ in range(max(1, a.warmup)):
+            b.run()
+        t0 = time.perf_counter()
+        dev = [b.run() for _ in range(a.steps)]
+        wall = (time.perf_counter() - t0) / a.steps
+        ex, _ = b.results()
+        zz[name] = ex[:, 2].copy()
+        info = b.info()
+        res[name] = {"circuits_per_s": round(a.count / wall, 1), "s_per_batch": round(wall, 6),
+                     "device_ms": round(min(dev), 3), "rounds": info["rounds"], "launches": info["launches"],
+                     "device_bytes": info["device_bytes"], "build_s": round(build_s, 3),
+                     "autotune_s": round(tune_s, 3), "tiles": tiles,
+                     "max_norm_error": float(np.max(np.abs(ex[:, 0] - 1)))}
+        b.close()
+        e.close()
+    # plain FP64, one circuit at a time (serial per-gate kernels), evenly spaced sample
+    idx = list(range(0, a.count, max(1, a.count // max(1, a.plain_sample))))[: a.plain_sample]
+    t0 = time.perf_counter()
+    pz = []
+    for i in idx:
+        n, d, s = specs[i]
+        c = build_circuit(hq, "hea", n, d, s)
+        c.set_option("mode", hq.HQ_MODE_SERIAL)
+        c.run(copy_back=False, destroy=False)
+        z, zzv = c.state_view(hq.HQ_C128).expect_z()
+        pz.append(float(zzv.sum()))
+        c.close()
+    plain_s = time.perf_counter() - t0
+    plain_cps = len(idx) / plain_s
+    dzz_plain = float(np.max(np.abs(np.array(pz) - zz["c128"][idx])))
+    fid_idx = sorted(set([int(np.argmax([s[0] for s in specs]))] + idx[:4]))
+    fids = [_fidelity(hq, specs[i]) for i in fid_idx]
+    out = {
+        "metric": "batched HEA circuits/s (complex64) vs plain FP64 one-at-a-time",
+        "value": res["c64"]["circuits_per_s"],
+        "unit": "circuits/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(res["c64"]["s_per_batch"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{a.count} hardware-efficient-ansatz circuits (RY, RZ per qubit, CZ ladder), "
+                               f"qubits round(exp(U[ln10, ln22])), depth U[4, 40], seed {a.seed}, "
+                               f"output sum <Z_i Z_(i+1)>", "parallelism": "single GPU"},
+        "batch": res,
+        "plain_fp64": {"circuits_per_s": round(plain_cps, 2), "sample": f"{len(idx)} evenly spaced circuits "
+                       f"({plain_s:.1f} s)", "max_abs_dzz_vs_batch_c128": dzz_plain},
+        "speedup_c64_vs_plain": round(res["c64"]["circuits_per_s"] / plain_cps, 2),
+        "speedup_c128_vs_plain": round(res["c128"]["circuits_per_s"] / plain_cps, 2),
+        "max_abs_dzz_c64_vs_c128": float(np.max(np.abs(zz["c64"] - zz["c128"]))),
+        "one_minus_fidelity_c64": {"max": max(fids), "circuits": fid_idx},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main_serve(a, hq):
+    """Serving form: Poisson stream per rate; complex64 engine (continuous
+    batching, HBM admission under 280 GB) vs plain FP64 one-at-a-time."""
+    import numpy as np
+    from hyquas_amd.serving import build_circuit, request_stream, serve, serve_plain, summarize
+    rates = [float(x) for x in a.rates.split(",") if x]
+    sweep = []
+    for k, rate in enumerate(rates):
+        reqs = request_stream(a.requests, rate, a.seed + k)
+        e = hq.Engine(hq.HQ_C64, 280 * 10 ** 9)
+        circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in reqs]
+        recs, wall = serve(hq, e, reqs, circuits)
+        st = e.stats()
+        e.close()
+        circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in reqs]
+        precs, pwall = serve_plain(hq, reqs, circuits)
+        dz = [abs(x["result"]["z"] - y["result"]["z"]) for x, y in zip(recs, precs) if not x["error"]]
+        topk_same = sum(1 for x, y, r in zip(recs, precs, reqs) if r["output"] == "topk" and not x["error"]
+                        and x["result"]["topk"][0][:4] == list(y["result"]["topk"][0][:4]))
+        sweep.append({"rate": rate, "engine_c64": summarize(recs, wall), "plain_fp64": summarize(precs, pwall),
+                      "peak_bytes": st["bytes_peak"], "rounds": st["rounds"],
+                      "max_abs_dz_c64_vs_plain": max(dz) if dz else None,
+                      "topk_top4_agree": f"{topk_same}/{sum(1 for r in reqs if r['output'] == 'topk')}"})
+    top = sweep[-1]
+    out = {
+        "metric": "served circuits/s + p99 vs plain",
+        "value": top["engine_c64"]["circuits_per_s"],
+        "unit": "circuits/s",
+        "n_gpus": 1, "steps": len(rates), "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Poisson stream, {a.requests} requests per rate, random-layer or ansatz "
+                               f"(even odds), qubits round(exp(U[ln12, ln28])), depth U[8, 60], output sum <Z_i> "
+                               f"or top-16 (even odds), seed {a.seed}, HBM cap 280 GB",
+                   "rates": rates, "parallelism": "single GPU"},
+        "sweep": sweep,
+        "speedup_vs_plain_at_top_rate": (round(top["engine_c64"]["circuits_per_s"] /
+                                               top["plain_fp64"]["circuits_per_s"], 2)
+                                         if top["plain_fp64"]["circuits_per_s"] else None),
     }
     print(json.dumps(out), flush=True)
 

@@ -203,7 +203,8 @@ __global__ __launch_bounds__(256) void k_topk(const TopkItem* __restrict__ items
 // ---- launch dispatch -----------------------------------------------------------------------
 
 #define HQ_BATCH_SHAPES(X) \
-    X(1, 6) X(2, 6) X(3, 6) X(4, 6) X(5, 6) X(2, 7) X(3, 7) X(4, 7) X(5, 7) X(3, 8) X(4, 8) X(5, 8)
+    X(1, 6) X(2, 6) X(3, 6) X(4, 6) X(5, 6) X(2, 7) X(3, 7) X(4, 7) X(5, 7) X(3, 8) X(4, 8) X(5, 8) X(3, 9) \
+    X(4, 9)
 
 template <typename R>
 static hq_status launch_batch_shape(int rb, int tb, const BatchItem* items, uint32_t count, uint64_t blocks,

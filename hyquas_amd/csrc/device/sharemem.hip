@@ -44,14 +44,8 @@ static hq_status launch_variant(hq_state s, hq_program p)
 {
     uint64_t blocks = (uint64_t)1 << (p->n - p->tile);
     const uint32_t mode = p->plan && p->plan->init ? TILE_INIT : 0u;
-    // non-temporal accesses (HQ_SHM_NT=1) stream the state past the caches
-    static const bool nt = getenv("HQ_SHM_NT") && atoi(getenv("HQ_SHM_NT")) != 0;
-    if (nt)
-        hipLaunchKernelGGL((k_sharemem<R, RB, TB, true>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
-                           (cplx<R>*)s->data, (const char*)p->d_prog, mode);
-    else
-        hipLaunchKernelGGL((k_sharemem<R, RB, TB, false>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
-                           (cplx<R>*)s->data, (const char*)p->d_prog, mode);
+    hipLaunchKernelGGL((k_sharemem<R, RB, TB, false>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
+                       (cplx<R>*)s->data, (const char*)p->d_prog, mode);
     HQ_HIP(hipGetLastError());
     return HQ_OK;
 }
@@ -64,7 +58,7 @@ static hq_status launch_by_shape(hq_state s, hq_program p)
     if (rb == RBV && tb == TBV) return launch_variant<R, RBV, TBV>(s, p);
     HQ_SHM_CASE(1, 6) HQ_SHM_CASE(2, 6) HQ_SHM_CASE(3, 6) HQ_SHM_CASE(4, 6) HQ_SHM_CASE(5, 6)
     HQ_SHM_CASE(2, 7) HQ_SHM_CASE(3, 7) HQ_SHM_CASE(4, 7) HQ_SHM_CASE(5, 7)
-    HQ_SHM_CASE(3, 8) HQ_SHM_CASE(4, 8) HQ_SHM_CASE(5, 8)
+    HQ_SHM_CASE(3, 8) HQ_SHM_CASE(4, 8) HQ_SHM_CASE(5, 8) HQ_SHM_CASE(3, 9) HQ_SHM_CASE(4, 9)
 #undef HQ_SHM_CASE
     set_error("sharemem: no kernel for this register/thread split");
     return HQ_ERR_UNSUPPORTED;
@@ -73,7 +67,7 @@ static hq_status launch_by_shape(hq_state s, hq_program p)
 static bool shape_supported(int rb, int tb)
 {
     static const int shapes[][2] = {{1, 6}, {2, 6}, {3, 6}, {4, 6}, {5, 6}, {2, 7}, {3, 7},
-                                    {4, 7}, {5, 7}, {3, 8}, {4, 8}, {5, 8}};
+                                    {4, 7}, {5, 7}, {3, 8}, {4, 8}, {5, 8}, {3, 9}, {4, 9}};
     for (auto& sh : shapes)
         if (sh[0] == rb && sh[1] == tb) return true;
     return false;

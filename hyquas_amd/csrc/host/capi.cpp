@@ -128,6 +128,8 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
         if (v == 1) {
             C.params.localSize = 10;
             C.params.coalesce = 3;
+            C.params.starts = 1;
+            C.params.beam = 1;
             C.params.minMat = 4;
             C.params.blasMat = 6;
             C.params.matLo = 4;
@@ -159,6 +161,12 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
     } else if (k == "window") {
         HQ_CARG(v >= 1 && v <= 4096, "window must be in [1, 4096]");
         C.params.window = (int)v;
+    } else if (k == "starts") {
+        HQ_CARG(v >= 1 && v <= 64, "starts must be in [1, 64]");
+        C.params.starts = (int)v;
+    } else if (k == "beam") {
+        HQ_CARG(v >= 1 && v <= 64, "beam must be in [1, 64]");
+        C.params.beam = (int)v;
     } else if (k == "jit") {
         HQ_CARG(v >= -1 && v <= 1, "jit must be -1 (auto), 0 or 1");
         C.exec.jit = (int)v;

@@ -58,8 +58,9 @@ void GroupFinder::remove(const std::vector<int>& idx)
     remain_.swap(out);
 }
 
-std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>& related, bool enableGlobal,
-                                   int localSize, qindex localQubits) const
+std::vector<std::vector<int>> GroupFinder::findAll(std::vector<char>& full, std::vector<qindex>& related,
+                                                   bool enableGlobal, int localSize, qindex localQubits,
+                                                   int starts) const
 {
     const int n = n_;
     const int gate_num = std::min<int>((int)remain_.size(), window_);
@@ -102,36 +103,64 @@ std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>&
         }
     }
 
-    // 2. greedily merge the longest chains that fit
-    std::vector<char> blocked(n, 0);
-    qindex selRel = 0;
-    while (true) {
-        int mx = 0, best = -1;
-        for (int i = 0; i < n; i++) {
-            int cnt = cur[i].count();
-            if (!blocked[i] && cnt > mx) {
-                if (popc(selRel | related[i]) <= localSize) { mx = cnt; best = i; }
-                else blocked[i] = 1;
+    // 2. greedily merge the longest chains that fit.  With starts > 1 the
+    // merge is repeated with each of the `starts` longest chains taken first,
+    // and the largest group wins (a one-group lookahead; starts = 1 is the
+    // reference's rule).
+    auto merge = [&](int first, Bits& sel) {
+        std::vector<Bits> cc = cur;
+        std::vector<char> blocked(n, 0);
+        qindex selRel = 0;
+        bool forced = first >= 0;
+        while (true) {
+            int mx = 0, best = -1;
+            if (forced) {
+                best = first;
+                mx = cc[first].count();
+                forced = false;
+            } else {
+                for (int i = 0; i < n; i++) {
+                    int cnt = cc[i].count();
+                    if (!blocked[i] && cnt > mx) {
+                        if (popc(selRel | related[i]) <= localSize) { mx = cnt; best = i; }
+                        else blocked[i] = 1;
+                    }
+                }
             }
-        }
-        if (mx == 0) break;
-        selected.orWith(cur[best]);
-        selRel |= related[best];
-        blocked[best] = 1;
-        for (int i = 0; i < n; i++) {
-            if (!blocked[i] && cur[i].any()) {
-                if ((related[i] | selRel) == selRel) {
-                    selected.orWith(cur[i]);
-                    blocked[i] = 1;
-                } else {
-                    cur[i].andNot(cur[best]);
+            if (mx == 0) break;
+            sel.orWith(cc[best]);
+            selRel |= related[best];
+            blocked[best] = 1;
+            for (int i = 0; i < n; i++) {
+                if (!blocked[i] && cc[i].any()) {
+                    if ((related[i] | selRel) == selRel) {
+                        sel.orWith(cc[i]);
+                        blocked[i] = 1;
+                    } else {
+                        cc[i].andNot(cc[best]);
+                    }
                 }
             }
         }
+    };
+    std::vector<Bits> cands;
+    merge(-1, selected);
+    cands.push_back(selected);
+    if (starts > 1) {
+        std::vector<int> order;
+        for (int i = 0; i < n; i++)
+            if (cur[i].any() && popc(related[i]) <= localSize) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cur[a].count() > cur[b].count(); });
+        for (int k = 1; k < starts && k < (int)order.size(); k++) {
+            Bits sel(gate_num);
+            merge(order[k], sel);
+            cands.push_back(sel);
+        }
     }
+    std::vector<char> blocked(n, 0);
 
     // 3. diagonal gates that commute into the group (OShareMem only)
-    if (enableGlobal) {
+    auto sweep = [&](Bits& sel) {
         std::fill(blocked.begin(), blocked.end(), 0);
         int cnt = 0;
         for (int id = 0; id < gate_num; id++) {
@@ -141,14 +170,14 @@ std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>&
                 for (int i = 0; i < n; i++) if (!blocked[i]) live = true;
                 if (!live) break;
             }
-            if (selected.test(id)) continue;
+            if (sel.test(id)) continue;
             const Gate& g = gates_[remain_[id]];
             if (g.isDiagonal()) {
                 if (g.isControlGate()) {
-                    if (!blocked[g.controlQubit] && !blocked[g.targetQubit]) selected.set(id);
+                    if (!blocked[g.controlQubit] && !blocked[g.targetQubit]) sel.set(id);
                     else blocked[g.controlQubit] = blocked[g.targetQubit] = 1;
                 } else {
-                    if (!blocked[g.targetQubit]) selected.set(id);
+                    if (!blocked[g.targetQubit]) sel.set(id);
                 }
             } else {
                 if (g.isControlGate()) blocked[g.controlQubit] = 1;
@@ -156,15 +185,31 @@ std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>&
                 blocked[g.targetQubit] = 1;
             }
         }
+    };
+    std::vector<std::vector<int>> out;
+    for (auto& sel : cands) {
+        if (enableGlobal) sweep(sel);
+        std::vector<int> ret;
+        for (int id = 0; id < gate_num; id++)
+            if (sel.test(id)) ret.push_back(remain_[id]);
+        bool dup = false;
+        for (auto& o : out) if (o == ret) dup = true;
+        if (!dup) out.push_back(std::move(ret));
     }
-
-    std::vector<int> ret;
-    for (int id = 0; id < gate_num; id++)
-        if (selected.test(id)) ret.push_back(remain_[id]);
-    return ret;
+    return out;
 }
 
-Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p)
+std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>& related, bool enableGlobal,
+                                   int localSize, qindex localQubits, int starts) const
+{
+    auto c = findAll(full, related, enableGlobal, localSize, localQubits, starts);
+    size_t best = 0;
+    for (size_t i = 1; i < c.size(); i++)
+        if (c[i].size() > c[best].size()) best = i;
+    return c.empty() ? std::vector<int>() : c[best];
+}
+
+static Schedule greedySchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p, double* cost)
 {
     Schedule sch;
     LocalGroup lg;
@@ -178,18 +223,20 @@ Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerPa
 
     std::vector<char> full(n);
     std::vector<qindex> related(n);
+    *cost = 0;
     while (!finder.empty()) {
         std::vector<int> ggIdx;
         Backend backend = Backend::PerGate;
         qindex cacheRelated = 0;
         double bestEff = 1e10;
+        int blasMs = 0;
         if (usePerGate) {
             std::fill(full.begin(), full.end(), 0);
             qindex co = 0;
             for (int j = 0; j < coalesce; j++) co |= qindex(1) << state.layout[j];
             std::fill(related.begin(), related.end(), co);
             cacheRelated = co;
-            ggIdx = finder.find(full, related, true, localSize, all);
+            ggIdx = finder.find(full, related, true, localSize, all, std::max(1, p.starts));
             backend = Backend::PerGate;
             if (useBLAS && !ggIdx.empty()) {
                 std::vector<const Gate*> gs;
@@ -209,6 +256,7 @@ Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerPa
                     ggIdx = idx;
                     backend = Backend::BLAS;
                     bestEff = eff;
+                    blasMs = ms;
                 }
             }
         } else if (useBLAS) {
@@ -216,8 +264,15 @@ Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerPa
             std::fill(related.begin(), related.end(), 0);
             ggIdx = finder.find(full, related, false, std::min(p.blasMat, n), all);
             backend = Backend::BLAS;
+            blasMs = std::min(p.blasMat, n);
         }
         if (ggIdx.empty()) break;   // cannot happen for valid parameters
+        {
+            std::vector<const Gate*> gs;
+            for (int x : ggIdx) gs.push_back(&gates[x]);
+            *cost += backend == Backend::PerGate ? p.cost.perfPerGate(n, p.precision, gs)
+                                                 : p.cost.perfBLAS(n, p.precision, blasMs);
+        }
         GateGroup gg;
         gg.backend = backend;
         for (int x : ggIdx) gg.addGate(gates[x], x, all, backend == Backend::PerGate);
@@ -231,4 +286,103 @@ Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerPa
     sch.localGroups.push_back(std::move(lg));
     sch.finalState = state;
     return sch;
+}
+
+
+// Beam search over OShareMem-only partitions: every step expands each kept
+// partial partition by the candidate groups of GroupFinder::findAll and keeps
+// the `beam` partitions that have placed the most gates; the first to place
+// every gate has the fewest groups found.  The layout never changes without
+// TransMM groups, so the coalesced qubits are physical 0..coalesce-1.
+static std::vector<std::vector<int>> beamPerGate(int n, const std::vector<Gate>& gates, const CompilerParams& p)
+{
+    struct Node {
+        GroupFinder f;
+        std::vector<std::vector<int>> groups;
+        size_t done;
+    };
+    const qindex all = n >= 63 ? ~qindex(0) : ((qindex(1) << n) - 1);
+    const int coalesce = std::min(p.coalesce, n);
+    const int localSize = std::min(p.localSize, n);
+    qindex co = 0;
+    for (int j = 0; j < coalesce; j++) co |= qindex(1) << j;
+    std::vector<Node> beam;
+    beam.push_back(Node{GroupFinder(n, gates, p.window), {}, 0});
+    std::vector<char> full(n);
+    std::vector<qindex> related(n);
+    const int width = std::max(1, p.beam);
+    const int starts = std::max(2, p.starts);
+    while (true) {
+        for (auto& nd : beam)
+            if (nd.f.empty()) return nd.groups;
+        std::vector<Node> next;
+        for (auto& nd : beam) {
+            std::fill(full.begin(), full.end(), 0);
+            std::fill(related.begin(), related.end(), co);
+            auto cands = nd.f.findAll(full, related, true, localSize, all, starts);
+            for (auto& c : cands) {
+                if (c.empty()) continue;
+                Node ch{nd.f, nd.groups, nd.done + c.size()};
+                ch.f.remove(c);
+                ch.groups.push_back(c);
+                next.push_back(std::move(ch));
+            }
+        }
+        if (next.empty()) return beam.front().groups;   // cannot happen for valid parameters
+        std::vector<size_t> ord(next.size());
+        for (size_t i = 0; i < ord.size(); i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return next[a].done > next[b].done; });
+        std::vector<Node> kept;
+        for (size_t i : ord) {
+            if ((int)kept.size() >= width) break;
+            bool dup = false;
+            for (auto& k : kept)
+                if (k.done == next[i].done && k.f.remain() == next[i].f.remain()) { dup = true; break; }
+            if (!dup) kept.push_back(std::move(next[i]));
+        }
+        beam.swap(kept);
+    }
+}
+
+static Schedule perGateSchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p,
+                                const std::vector<std::vector<int>>& groups, double* cost)
+{
+    Schedule sch;
+    LocalGroup lg;
+    State state(n);
+    const qindex all = n >= 63 ? ~qindex(0) : ((qindex(1) << n) - 1);
+    qindex co = 0;
+    for (int j = 0; j < std::min(p.coalesce, n); j++) co |= qindex(1) << j;
+    *cost = 0;
+    for (auto& idx : groups) {
+        std::vector<const Gate*> gs;
+        for (int x : idx) gs.push_back(&gates[x]);
+        *cost += p.cost.perfPerGate(n, p.precision, gs);
+        GateGroup gg;
+        gg.backend = Backend::PerGate;
+        for (int x : idx) gg.addGate(gates[x], x, all, true);
+        gg.relatedQubits |= co;
+        state = gg.initState(state, n, p.minMat);
+        lg.relatedQubits |= gg.relatedQubits;
+        lg.fullGroups.push_back(std::move(gg));
+    }
+    lg.state = State(n);
+    sch.localGroups.push_back(std::move(lg));
+    sch.finalState = state;
+    return sch;
+}
+
+Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p)
+{
+    double greedyCost = 0;
+    Schedule g = greedySchedule(n, gates, p, &greedyCost);
+    if (p.beam <= 1 || (p.mode != HQ_MODE_GROUP && p.mode != HQ_MODE_MIX)) return g;
+    double beamCost = 0;
+    Schedule b = perGateSchedule(n, gates, p, beamPerGate(n, gates, p), &beamCost);
+    size_t ng = 0, nb = 0;
+    for (auto& lg : g.localGroups) ng += lg.fullGroups.size();
+    for (auto& lg : b.localGroups) nb += lg.fullGroups.size();
+    // OShareMem-only: fewer groups wins; mix: the lower predicted time
+    if (p.mode == HQ_MODE_GROUP ? nb < ng : beamCost < greedyCost) return b;
+    return g;
 }

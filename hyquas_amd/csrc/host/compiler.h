@@ -15,6 +15,8 @@ struct CompilerParams {
     int blasMat = 6;          // BLAS_MAT_LIMIT (HQ_MODE_BLAS)
     int matLo = 4, matHi = 8; // mix mode tries matLo <= matSize < matHi
     int window = 512;         // gates considered per group (OneLayerCompiler<512>)
+    int starts = 4;           // OShareMem merge lookahead: 1 = reference rule
+    int beam = 8;             // > 1: beam search over OShareMem-only partitions (kept if better)
     CostModel cost;
 };
 
@@ -26,7 +28,11 @@ public:
     // Selected gate indices (ascending) for a group with at most localSize
     // related qubits.  full/related are updated like the reference's arrays.
     std::vector<int> find(std::vector<char>& full, std::vector<qindex>& related, bool enableGlobal, int localSize,
-                          qindex localQubits) const;
+                          qindex localQubits, int starts = 1) const;
+    // Candidate groups: the reference's merge plus one per forced first chain
+    // among the `starts` longest (distinct groups only, reference's first).
+    std::vector<std::vector<int>> findAll(std::vector<char>& full, std::vector<qindex>& related, bool enableGlobal,
+                                          int localSize, qindex localQubits, int starts) const;
     void remove(const std::vector<int>& idx);
     bool empty() const { return remain_.empty(); }
     const std::vector<int>& remain() const { return remain_; }

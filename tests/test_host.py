@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import (CCX, CNOT, H, RX, hea_layers, random_layers, random_mixed, to_oracle)
-from test_oracle import QASM, NPAR, _check_partition_invariants
+from helpers import (CCX, CNOT, H, RX, hea_layers, random_layers, random_mixed, rel_err, to_oracle)
+from test_oracle import QASM, NPAR, _check_partition_invariants, _execute_partition
 
 ALL_TYPES = list(range(22)) + [23, 24, 25, 26, 27]
 CTRL = {0: 2, 1: 1, 2: 1, 3: 1, 4: 1, 5: 1, 6: 1, 7: 1}
@@ -110,6 +110,33 @@ def test_gfx950_partition_invariants(hq, share):
     spec = random_mixed(n, 400, 11, ccx=True)
     pg, _ = _product_partition(hq, n, spec, 4, reference=False, share_qubits=share)
     _check_partition_invariants(spec, n, pg, share, 7)
+
+
+@pytest.mark.parametrize("n,share,kind", [(14, 8, "layers"), (16, 9, "mixed"), (18, 10, "hea")])
+def test_beam_partition_is_valid_and_not_larger(hq, n, share, kind):
+    """Beam search over OShareMem-only partitions: every gate once, per-qubit
+    order kept, tiles within the limit, the same state as the circuit on the
+    oracle, and never more groups than the greedy rule."""
+    spec = (random_layers(n, 10, 3) if kind == "layers" else
+            random_mixed(n, 300, 4, ccx=True) if kind == "mixed" else hea_layers(n, 6, 5))
+    greedy, _ = _product_partition(hq, n, spec, 1, reference=False, share_qubits=share, beam=1, starts=1)
+    beam, _ = _product_partition(hq, n, spec, 1, reference=False, share_qubits=share, beam=8, starts=4)
+    _check_partition_invariants(spec, n, beam, share, 7)
+    assert len(beam) <= len(greedy)
+    got, pos = _execute_partition(spec, n, [dict(g, perm=None) for g in beam])
+    assert pos == list(range(n))
+    want = oracle.run_plain(to_oracle(spec), n)
+    assert rel_err(got, want) <= 1e-12
+
+
+def test_beam_search_reduces_30q_groups(hq):
+    c = hq.Circuit(30).add_random_layers(30, 0)
+    c.partition()
+    beam = len(c.schedule())
+    c.set_option("beam", 1)
+    c.set_option("starts", 1)
+    c.partition()
+    assert beam < len(c.schedule())
 
 
 def test_ccx_second_control_is_a_dependency(hq):

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 @pytest.mark.parametrize("n,k,seed,prec,rb", [(10, 9, 31, 0, 0), (11, 10, 32, 0, 3), (12, 11, 33, 1, 0),
-                                              (11, 11, 34, 0, 5)])
+                                              (11, 11, 34, 0, 5), (14, 13, 35, 0, 4)])
 def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb):
     rs = np.random.default_rng(seed)
     rest = rs.choice(np.arange(3, n), k - 3, replace=False)
@@ -41,3 +41,16 @@ def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb):
     got = np.fromfile(tmp_path / "out.bin", dtype=dt)
     want = oracle.run_plain(g, n, psi0)
     assert rel_err(got, want) <= (1e-13 if prec == 0 else 2e-6)
+
+
+def test_exchange_barriers(hq):
+    # every LDS exchange after the first is preceded by a barrier that orders the
+    # previous exchange's reads before its writes: 2 * (stages - 1) - 1 barriers
+    n, mask = 12, (1 << 12) - 1
+    g = _group_for_mask(n, mask, 120, 7)
+    src, _, _ = hq.sharemem_source(n, g, mask)
+    exchanges = src.count("lds[sw ^")
+    stages = src.count("r0 = lds[") + 1
+    assert stages >= 3
+    assert src.count("__syncthreads") == 2 * (stages - 1) - 1
+    assert exchanges > 0
