@@ -90,7 +90,7 @@ def main():
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
                     help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", choices=["sim", "batch", "serve"], default="sim",
+    ap.add_argument("--form", choices=["sim", "batch", "serve", "precision"], default="sim",
                     help="sim: one 30-qubit circuit (default); batch: 4096 ansatz circuits; "
                          "serve: Poisson request stream")
     ap.add_argument("--count", type=int, default=4096, help="batch: circuits")
@@ -116,6 +116,8 @@ def main():
         return main_batch(a, hq)
     if a.form == "serve":
         return main_serve(a, hq)
+    if a.form == "precision":
+        return main_precision(a, hq)
 
     n, layers = a.qubits, a.layers
     prec = hq.HQ_C128 if a.precision == "c128" else hq.HQ_C64
@@ -348,6 +350,106 @@ def main_serve(a, hq):
         "speedup_vs_plain_at_top_rate": (round(top["engine_c64"]["circuits_per_s"] /
                                                top["plain_fp64"]["circuits_per_s"], 2)
                                          if top["plain_fp64"]["circuits_per_s"] else None),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _timed_run(hq, c, steps, warmup):
+    for _ in range(warmup):
+        c.run(copy_back=False, destroy=False)
+    t0 = time.perf_counter()
+    launches = []
+    for _ in range(steps):
+        c.run(copy_back=False, destroy=False)
+        launches.extend(c.launch_times())
+    return (time.perf_counter() - t0) / steps, launches
+
+
+def main_precision(a, hq):
+    """Precision form: random layers in complex64 at 33 qubits (64 GiB) and 34
+    qubits (128 GiB), 24 layers; the TransMM path (FP64 fusion rounded to FP32,
+    FP32 MFMA) at 30 qubits; max |d amp| and 1 - fidelity of complex64 against a
+    complex128 run of the 30-qubit instance of the same circuit."""
+    import numpy as np
+    layers = 24
+    out_runs = {}
+    # 33 and 34 qubits, complex64, OShareMem groups
+    for n, steps, warm in ((33, a.steps if a.steps < 5 else 3, 1), (34, 1, 0)):
+        c = hq.Circuit(n).add_random_layers(layers, a.seed)
+        c.set_option("precision", hq.HQ_C64)
+        c.set_option("mode", hq.HQ_MODE_GROUP)
+        c.set_option("profile_events", 1)
+        t0 = time.perf_counter()
+        c.compile()
+        comp = time.perf_counter() - t0
+        sec, launches = _timed_run(hq, c, steps, warm)
+        shm = [ms for k, ms in launches if k == hq.KIND_SHAREMEM]
+        per_step = len(shm) // steps
+        alg = 2.0 * (1 << n) * 8 * (per_step - 0.5) / per_step
+        avg = sum(shm) / len(shm)
+        norm = c.state_view(hq.HQ_C64).norm2()
+        out_runs[f"{n}q_c64"] = {"s_per_sim": round(sec, 4), "groups": per_step, "compile_s": round(comp, 2),
+                                 "avg_launch_ms": round(avg, 3), "achieved_GBps": round(alg / avg / 1e6, 1),
+                                 "hbm_frac": round(alg / avg / 1e6 / HBM_PEAK_GBS, 4), "norm_error": abs(norm - 1),
+                                 "state_GiB": (1 << n) * 8 / 2 ** 30}
+        c.close()
+    # 30 qubits: complex128 reference, complex64 OShareMem, complex64 TransMM (FP32 MFMA)
+    n = 30
+    ref = hq.Circuit(n).add_random_layers(layers, a.seed)
+    ref.set_option("precision", hq.HQ_C128)
+    ref.set_option("mode", hq.HQ_MODE_GROUP)
+    ref.run(copy_back=False, destroy=False)
+    ref_state = ref.state_view(hq.HQ_C128)
+    psi_ref = ref_state.download()
+    errs = {}
+    for name, mode, mat in (("c64_group", hq.HQ_MODE_GROUP, 0), ("c64_transmm_k64", hq.HQ_MODE_BLAS, 6),
+                            ("c64_transmm_k128", hq.HQ_MODE_BLAS, 7)):
+        c = hq.Circuit(n).add_random_layers(layers, a.seed)
+        c.set_option("precision", hq.HQ_C64)
+        c.set_option("mode", mode)
+        c.set_option("profile_events", 1)
+        if mat:
+            c.set_option("blas_mat", mat)
+        c.compile()
+        sec, launches = _timed_run(hq, c, 1, 1 if mode == hq.HQ_MODE_GROUP else 0)
+        if mode == hq.HQ_MODE_GROUP:
+            ov = ref_state.inner(c.state_view(hq.HQ_C64))
+            psi = c.state_view(hq.HQ_C64).download()
+        else:
+            psi = c.logical_state(hq.HQ_C64)
+            st = hq.State(n, hq.HQ_C64)
+            st.upload(psi)
+            ov = ref_state.inner(st)
+            st.close()
+        d = float(np.max(np.abs(psi.astype(np.complex128) - psi_ref)))
+        n64 = float(np.vdot(psi, psi).real)
+        dense = [ms for k, ms in launches if k == hq.KIND_DENSE]
+        tr = [ms for k, ms in launches if k == hq.KIND_TRANSPOSE]
+        # raw: |<ref|psi>|^2 with psi as computed; normalized: psi scaled to unit norm first
+        rec = {"s_per_sim": round(sec, 4), "max_abs_damp": d, "one_minus_fidelity": 1.0 - abs(ov) ** 2,
+               "one_minus_fidelity_normalized": 1.0 - abs(ov) ** 2 / n64, "norm_error": abs(n64 - 1.0)}
+        if dense:
+            K = 1 << mat
+            fl = 8.0 * K * (1 << n)
+            rec.update({"dense_launches": len(dense), "dense_avg_ms": round(sum(dense) / len(dense), 3),
+                        "dense_TFLOPs": round(fl / (sum(dense) / len(dense)) / 1e9, 2),
+                        "mfma_frac_fp32": round(fl / (sum(dense) / len(dense)) / 1e9 / 157.3, 4),
+                        "transpose_launches": len(tr),
+                        "transpose_avg_ms": round(sum(tr) / len(tr), 3) if tr else None})
+        errs[name] = rec
+        del psi
+        c.close()
+    ref.close()
+    top = out_runs["33q_c64"]
+    out = {
+        "metric": "33q complex64 sim time",
+        "value": top["s_per_sim"], "unit": "s", "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": round(top["s_per_sim"] * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"random-layer circuit, {layers} layers, seed {a.seed}, complex64; errors of the "
+                               f"30-qubit instance against complex128", "parallelism": "single GPU"},
+        "runs": out_runs,
+        "errors_vs_c128_30q": errs,
     }
     print(json.dumps(out), flush=True)
 

@@ -76,6 +76,14 @@ enum ShmClass : uint8_t {
     SHM_GEN = 4,    // complex 2x2                     (U2, U3, ...)
     SHM_DIAG = 5,   // lo *= d0, hi *= d1              (RZ, CRZ, ...)
     SHM_PHASE = 6,  // hi *= d1                        (T, S, Z, U1, CZ, CU1, ...)
+    // uncontrolled gates with a scalar factored out (the group's product of
+    // factors is applied once, by a final SHM_SCALE op); t, u = m[0]
+    SHM_HADD = 7,   // lo' = lo + hi, hi' = lo - hi               (H / r)
+    SHM_ROTA = 8,   // lo' = lo - t hi, hi' = t lo + hi           (RY / cos)
+    SHM_ROTB = 9,   // lo' = u lo - hi, hi' = lo + u hi           (RY / sin)
+    SHM_RXA = 10,   // lo' = lo + i t hi, hi' = i t lo + hi       (RX / cos)
+    SHM_RXB = 11,   // lo' = u lo + i hi, hi' = i lo + u hi       (RX / sin)
+    SHM_SCALE = 12, // every amplitude *= (m[0], m[1])
 };
 
 enum ShmWhere : uint8_t { SHM_IN_REG = 0, SHM_IN_THREAD = 1, SHM_IN_BLOCK = 2 };

@@ -42,6 +42,25 @@ __device__ __forceinline__ void pair_update(cplx<R>& lo, cplx<R>& hi, const R* _
     } else if constexpr (CLS == SHM_DIAG) {
         lo.x = m[0] * lr - m[1] * li; lo.y = m[0] * li + m[1] * lr;
         hi.x = m[6] * hr - m[7] * hm; hi.y = m[6] * hm + m[7] * hr;
+    } else if constexpr (CLS == SHM_HADD) {
+        lo.x = lr + hr; lo.y = li + hm;
+        hi.x = lr - hr; hi.y = li - hm;
+    } else if constexpr (CLS == SHM_ROTA) {
+        const R t = m[0];
+        lo.x = lr - t * hr; lo.y = li - t * hm;
+        hi.x = hr + t * lr; hi.y = hm + t * li;
+    } else if constexpr (CLS == SHM_ROTB) {
+        const R u = m[0];
+        lo.x = u * lr - hr; lo.y = u * li - hm;
+        hi.x = lr + u * hr; hi.y = li + u * hm;
+    } else if constexpr (CLS == SHM_RXA) {
+        const R t = m[0];
+        lo.x = lr - t * hm; lo.y = li + t * hr;
+        hi.x = hr - t * li; hi.y = hm + t * lr;
+    } else if constexpr (CLS == SHM_RXB) {
+        const R u = m[0];
+        lo.x = u * lr - hm; lo.y = u * li + hr;
+        hi.x = u * hr - li; hi.y = u * hm + lr;
     } else {  // SHM_PHASE
         hi.x = m[6] * hr - m[7] * hm; hi.y = m[6] * hm + m[7] * hr;
     }

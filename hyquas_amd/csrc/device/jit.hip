@@ -184,7 +184,9 @@ struct Emitter {
             }
         }
         const bool rename = identity_ok_renaming && !blockc;
-        if (op.where == SHM_IN_REG) {
+        if (op.cls == SHM_SCALE) {
+            for (int j = 0; j < NR; j++) scale_stmt(j, mv[0], mv[1]);
+        } else if (op.where == SHM_IN_REG) {
             const int Pb = op.tpos;
             for (int j = 0; j < NR; j++) {
                 if (j & (1 << Pb)) continue;
@@ -223,6 +225,30 @@ struct Emitter {
                 case SHM_DIAG:
                     scale_stmt(j, mv[0], mv[1]);
                     scale_stmt(jh, mv[6], mv[7]);
+                    break;
+                case SHM_HADD:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = lo.x + hi.x; "
+                      << r(j) << ".y = lo.y + hi.y; " << r(jh) << ".x = lo.x - hi.x; " << r(jh) << ".y = lo.y - hi.y; }\n";
+                    break;
+                case SHM_ROTA:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = lo.x - " << mv[0]
+                      << " * hi.x; " << r(j) << ".y = lo.y - " << mv[0] << " * hi.y; " << r(jh) << ".x = hi.x + "
+                      << mv[0] << " * lo.x; " << r(jh) << ".y = hi.y + " << mv[0] << " * lo.y; }\n";
+                    break;
+                case SHM_ROTB:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = " << mv[0]
+                      << " * lo.x - hi.x; " << r(j) << ".y = " << mv[0] << " * lo.y - hi.y; " << r(jh) << ".x = lo.x + "
+                      << mv[0] << " * hi.x; " << r(jh) << ".y = lo.y + " << mv[0] << " * hi.y; }\n";
+                    break;
+                case SHM_RXA:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = lo.x - " << mv[0]
+                      << " * hi.y; " << r(j) << ".y = lo.y + " << mv[0] << " * hi.x; " << r(jh) << ".x = hi.x - "
+                      << mv[0] << " * lo.y; " << r(jh) << ".y = hi.y + " << mv[0] << " * lo.x; }\n";
+                    break;
+                case SHM_RXB:
+                    o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = " << mv[0]
+                      << " * lo.x - hi.y; " << r(j) << ".y = " << mv[0] << " * lo.y + hi.x; " << r(jh) << ".x = "
+                      << mv[0] << " * hi.x - lo.y; " << r(jh) << ".y = " << mv[0] << " * hi.y + lo.x; }\n";
                     break;
                 default:  // SHM_PHASE
                     scale_stmt(jh, mv[6], mv[7]);

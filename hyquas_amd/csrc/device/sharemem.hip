@@ -22,6 +22,8 @@
 #include "sharemem_dev.h"
 
 #include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 
@@ -84,7 +86,8 @@ int sharemem_default_rb(int precision, int tile)
     // amplitudes per thread at the default 12/13-qubit tiles
     if (tile <= 10) return std::max(1, tile - 6);
     if (precision == HQ_C128) return tile >= 13 ? 5 : 4;
-    return tile >= 12 ? 5 : tile - 6;
+    // complex64: 256 threads x 16 amplitudes at 12 qubits measured fastest
+    return tile >= 13 ? 5 : tile == 12 ? 4 : tile - 6;
 }
 
 // ---- host side: lowering + register-stage planning -------------------------------------------
@@ -98,6 +101,45 @@ struct Lowered {
     int c_tile[2], c_block[2];
     double m[8];
 };
+
+// Factor a scalar out of an uncontrolled gate so that its pair update needs
+// half the multiplies: H = r [[1, 1], [1, -1]]; a rotation [[c, -s], [s, c]]
+// = c [[1, -t], [t, 1]] (t = s/c) or s [[u, -1], [1, u]] (u = c/s), whichever
+// keeps |t|, |u| <= 1; RX-type [[a, ib], [ib, a]] likewise; a unit diagonal
+// diag(d0, d1) = d0 diag(1, d1/d0).  Other gates are left as they are.
+void factor_scalar(Lowered& L, std::complex<double>* S)
+{
+    double* m = L.m;
+    if (L.cls == SHM_REAL) {
+        if (m[0] == m[2] && m[0] == m[4] && m[0] == -m[6] && m[0] != 0) {
+            *S *= m[0];
+            L.cls = SHM_HADD;
+            memset(m, 0, sizeof(L.m));
+        } else if (m[0] == m[6] && m[2] == -m[4] && (m[0] != 0 || m[4] != 0)) {
+            const double c = m[0], s = m[4];
+            if (std::fabs(c) >= std::fabs(s)) { *S *= c; L.cls = SHM_ROTA; memset(m, 0, sizeof(L.m)); m[0] = s / c; }
+            else { *S *= s; L.cls = SHM_ROTB; memset(m, 0, sizeof(L.m)); m[0] = c / s; }
+        }
+    } else if (L.cls == SHM_RX) {
+        // lo' = a lo + i b hi, hi' = i c lo + d hi with a = d, b = c
+        const double a = m[0], b = m[3];
+        if (a == m[6] && b == m[5] && (a != 0 || b != 0)) {
+            if (std::fabs(a) >= std::fabs(b)) { *S *= a; L.cls = SHM_RXA; memset(m, 0, sizeof(L.m)); m[0] = b / a; }
+            else { *S *= b; L.cls = SHM_RXB; memset(m, 0, sizeof(L.m)); m[0] = a / b; }
+        }
+    } else if (L.cls == SHM_DIAG) {
+        const std::complex<double> d0(m[0], m[1]), d1(m[6], m[7]);
+        if (std::fabs(std::abs(d0) - 1.0) < 1e-12) {
+            const std::complex<double> q = d1 / d0;
+            *S *= d0;
+            L.cls = SHM_PHASE;
+            memset(m, 0, sizeof(L.m));
+            m[0] = 1.0;
+            m[6] = q.real();
+            m[7] = q.imag();
+        }
+    }
+}
 
 // swizzle column of tile bit i (bits >= 3 fold into the low 3 bits)
 uint32_t swz_column(int i)
@@ -147,6 +189,10 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
     std::vector<Lowered> low;
     std::vector<int> low_type;
     low.reserve(num_gates);
+    // uncontrolled H, RX, RY and unit diagonals lose a scalar factor; the
+    // product S is applied once at the end of the pass (HQ_SHM_FACTOR=0: off)
+    static const bool factor = !getenv("HQ_SHM_FACTOR") || atoi(getenv("HQ_SHM_FACTOR")) != 0;
+    std::complex<double> S(1.0, 0.0);
     for (int i = 0; i < num_gates; i++) {
         const hq_gate& g = gates[i];
         hq_status st = validate_gate(g, n);
@@ -167,6 +213,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
             L.c_block[c] = cs[c] >= 0 ? block_of[cs[c]] : -1;
         }
         memcpy(L.m, g.mat, sizeof(L.m));
+        if (factor && g.control < 0 && g.control2 < 0) factor_scalar(L, &S);
         low.push_back(L);
         low_type.push_back(g.type);
     }
@@ -276,6 +323,16 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
         }
         d.op_end = (uint32_t)ops.size();
         dst.push_back(d);
+    }
+    if (S != std::complex<double>(1.0, 0.0)) {
+        ShmOp o;
+        memset(&o, 0, sizeof(o));
+        o.cls = SHM_SCALE;
+        o.where = SHM_IN_REG;
+        o.m[0] = S.real();
+        o.m[1] = S.imag();
+        ops.push_back(o);
+        dst.back().op_end = (uint32_t)ops.size();
     }
 
     ShmHeader hdr;

@@ -38,6 +38,11 @@ __device__ __forceinline__ void reg_op(cplx<R> (&a)[1 << RB], int cls, uint32_t 
     case SHM_RX: reg_pairs<R, RB, SHM_RX, P>(a, creg, m); break;
     case SHM_GEN: reg_pairs<R, RB, SHM_GEN, P>(a, creg, m); break;
     case SHM_DIAG: reg_pairs<R, RB, SHM_DIAG, P>(a, creg, m); break;
+    case SHM_HADD: reg_pairs<R, RB, SHM_HADD, P>(a, creg, m); break;
+    case SHM_ROTA: reg_pairs<R, RB, SHM_ROTA, P>(a, creg, m); break;
+    case SHM_ROTB: reg_pairs<R, RB, SHM_ROTB, P>(a, creg, m); break;
+    case SHM_RXA: reg_pairs<R, RB, SHM_RXA, P>(a, creg, m); break;
+    case SHM_RXB: reg_pairs<R, RB, SHM_RXB, P>(a, creg, m); break;
     default: reg_pairs<R, RB, SHM_PHASE, P>(a, creg, m); break;
     }
 }
@@ -56,6 +61,11 @@ __device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const ShmOp* __r
     const uint32_t creg = op->creg;
     const int cls = op->cls;
     const int tpos = op->tpos;
+    if (cls == SHM_SCALE) {
+#pragma unroll
+        for (int j = 0; j < (1 << RB); j++) scale(a[j], m[0], m[1]);
+        return;
+    }
     if (op->where == SHM_IN_REG) {
         switch (tpos) {
         case 0: reg_op<R, RB, 0>(a, cls, creg, m); break;

@@ -54,7 +54,8 @@ def test_sharemem_matches_oracle(gpu, n, k, seed):
     s.upload(psi0)
     p = s.program_sharemem(g, mask)
     tile, stages, ops = p.info()
-    assert tile == k and stages >= 1 and ops == len(g)
+    # one op per gate, plus the final scale of the factored scalars
+    assert tile == k and stages >= 1 and ops in (len(g), len(g) + 1)
     s.launch(p)
     assert rel_err(s.download(), want) <= 1e-12
 
