@@ -89,6 +89,7 @@ def main():
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
                     help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
+    ap.add_argument("--cost-model", default="", help="parameter file for the partitioner's cost model")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["sim", "batch", "serve", "precision"], default="sim",
                     help="sim: one 30-qubit circuit (default); batch: 4096 ansatz circuits; "
@@ -134,6 +135,8 @@ def main():
         c.set_option("coalesce", a.coalesce)
     c.set_option("jit", a.jit)
     jit_on = a.jit == 1 or (a.jit == -1 and n >= 22)
+    if a.cost_model:
+        c.load_cost_model(a.cost_model)
     t0 = time.perf_counter()
     c.compile()
     compile_s = time.perf_counter() - t0

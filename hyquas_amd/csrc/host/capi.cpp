@@ -130,6 +130,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
             C.params.coalesce = 3;
             C.params.starts = 1;
             C.params.beam = 1;
+            C.params.coalesceAuto = false;
             C.params.minMat = 4;
             C.params.blasMat = 6;
             C.params.matLo = 4;
@@ -143,6 +144,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
     } else if (k == "coalesce") {
         HQ_CARG(v >= 0 && v <= 8, "coalesce must be in [0, 8]");
         C.params.coalesce = (int)v;
+        C.params.coalesceAuto = false;
     } else if (k == "mat_min") {
         HQ_CARG(v >= 1 && v <= 7, "mat_min must be in [1, 7]");
         C.params.matLo = (int)v;

@@ -209,6 +209,20 @@ std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>&
     return c.empty() ? std::vector<int>() : c[best];
 }
 
+// Physical tile of an OShareMem group: its related (logical) qubits at their
+// physical positions, topped up with the lowest physical positions
+// (fillRelatedQubits, src/executor.cpp:585-596).
+static qindex tileMask(qindex relatedLogical, const State& st, int size)
+{
+    qindex m = 0;
+    int cnt = 0;
+    for (int q = 0; q < (int)st.pos.size(); q++)
+        if ((relatedLogical >> q) & 1) { m |= qindex(1) << st.pos[q]; cnt++; }
+    for (int ph = 0; ph < (int)st.pos.size() && cnt < size; ph++)
+        if (!((m >> ph) & 1)) { m |= qindex(1) << ph; cnt++; }
+    return m;
+}
+
 static Schedule greedySchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p, double* cost)
 {
     Schedule sch;
@@ -240,8 +254,10 @@ static Schedule greedySchedule(int n, const std::vector<Gate>& gates, const Comp
             backend = Backend::PerGate;
             if (useBLAS && !ggIdx.empty()) {
                 std::vector<const Gate*> gs;
-                for (int x : ggIdx) gs.push_back(&gates[x]);
-                bestEff = p.cost.perfPerGate(n, p.precision, gs) / ggIdx.size();
+                GateGroup tmp;
+                for (int x : ggIdx) { gs.push_back(&gates[x]); tmp.addGate(gates[x], x, all, true); }
+                bestEff = p.cost.perfPerGate(n, p.precision, gs,
+                                             tileMask(tmp.relatedQubits | co, state, localSize)) / ggIdx.size();
             }
         }
         if (usePerGate && useBLAS) {
@@ -267,16 +283,17 @@ static Schedule greedySchedule(int n, const std::vector<Gate>& gates, const Comp
             blasMs = std::min(p.blasMat, n);
         }
         if (ggIdx.empty()) break;   // cannot happen for valid parameters
-        {
-            std::vector<const Gate*> gs;
-            for (int x : ggIdx) gs.push_back(&gates[x]);
-            *cost += backend == Backend::PerGate ? p.cost.perfPerGate(n, p.precision, gs)
-                                                 : p.cost.perfBLAS(n, p.precision, blasMs);
-        }
         GateGroup gg;
         gg.backend = backend;
         for (int x : ggIdx) gg.addGate(gates[x], x, all, backend == Backend::PerGate);
         if (backend == Backend::PerGate) gg.relatedQubits |= cacheRelated;
+        {
+            std::vector<const Gate*> gs;
+            for (int x : ggIdx) gs.push_back(&gates[x]);
+            *cost += backend == Backend::PerGate
+                         ? p.cost.perfPerGate(n, p.precision, gs, tileMask(gg.relatedQubits, state, localSize))
+                         : p.cost.perfBLAS(n, p.precision, blasMs);
+        }
         state = gg.initState(state, n, p.minMat);
         finder.remove(ggIdx);
         lg.relatedQubits |= gg.relatedQubits;
@@ -354,14 +371,15 @@ static Schedule perGateSchedule(int n, const std::vector<Gate>& gates, const Com
     qindex co = 0;
     for (int j = 0; j < std::min(p.coalesce, n); j++) co |= qindex(1) << j;
     *cost = 0;
+    const int localSize = std::min(p.localSize, n);
     for (auto& idx : groups) {
         std::vector<const Gate*> gs;
         for (int x : idx) gs.push_back(&gates[x]);
-        *cost += p.cost.perfPerGate(n, p.precision, gs);
         GateGroup gg;
         gg.backend = Backend::PerGate;
         for (int x : idx) gg.addGate(gates[x], x, all, true);
         gg.relatedQubits |= co;
+        *cost += p.cost.perfPerGate(n, p.precision, gs, tileMask(gg.relatedQubits, state, localSize));
         state = gg.initState(state, n, p.minMat);
         lg.relatedQubits |= gg.relatedQubits;
         lg.fullGroups.push_back(std::move(gg));
@@ -372,17 +390,42 @@ static Schedule perGateSchedule(int n, const std::vector<Gate>& gates, const Com
     return sch;
 }
 
-Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p)
+static Schedule compileOne(int n, const std::vector<Gate>& gates, const CompilerParams& p, double* cost)
 {
     double greedyCost = 0;
     Schedule g = greedySchedule(n, gates, p, &greedyCost);
+    *cost = greedyCost;
     if (p.beam <= 1 || (p.mode != HQ_MODE_GROUP && p.mode != HQ_MODE_MIX)) return g;
     double beamCost = 0;
     Schedule b = perGateSchedule(n, gates, p, beamPerGate(n, gates, p), &beamCost);
-    size_t ng = 0, nb = 0;
-    for (auto& lg : g.localGroups) ng += lg.fullGroups.size();
-    for (auto& lg : b.localGroups) nb += lg.fullGroups.size();
-    // OShareMem-only: fewer groups wins; mix: the lower predicted time
-    if (p.mode == HQ_MODE_GROUP ? nb < ng : beamCost < greedyCost) return b;
+    if (beamCost < greedyCost) {
+        *cost = beamCost;
+        return b;
+    }
     return g;
+}
+
+Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerParams& p)
+{
+    double cost = 0;
+    const bool search = p.coalesceAuto && !p.cost.reference && (p.mode == HQ_MODE_GROUP || p.mode == HQ_MODE_MIX);
+    if (!search) return compileOne(n, gates, p, &cost);
+    // the coalesced (always-in-tile) low qubits trade run length against free
+    // tile slots: keep the partition with the lowest predicted time
+    Schedule best;
+    double bestCost = 0;
+    bool have = false;
+    for (int c = 2; c <= 5; c++) {
+        if (c >= std::min(p.localSize, n)) break;
+        CompilerParams q = p;
+        q.coalesce = c;
+        Schedule s = compileOne(n, gates, q, &cost);
+        if (!have || cost < bestCost) {
+            best = std::move(s);
+            bestCost = cost;
+            have = true;
+        }
+    }
+    if (!have) return compileOne(n, gates, p, &cost);
+    return best;
 }

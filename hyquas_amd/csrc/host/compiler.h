@@ -11,6 +11,7 @@ struct CompilerParams {
     int precision = HQ_C128;
     int localSize = 12;       // OShareMem tile qubits (LOCAL_QUBIT_SIZE)
     int coalesce = 4;         // COALESCE_GLOBAL: lowest physical qubits always in the tile (reference: 3)
+    bool coalesceAuto = true; // search coalesce in 2..5 by predicted time (off once "coalesce" is set)
     int minMat = 4;           // MIN_MAT_SIZE
     int blasMat = 6;          // BLAS_MAT_LIMIT (HQ_MODE_BLAS)
     int matLo = 4, matHi = 8; // mix mode tries matLo <= matSize < matHi

@@ -25,23 +25,39 @@ struct CostModel {
     double group_overhead_per_amp = 1.0 / (1 << 27);
 
     // --- MI355X profile (seconds, bytes) ---------------------------------------------
-    double hbm_bw = 5.6e12;         // achieved streaming bandwidth, B/s
-    double valu_rate = 39.3e12 * 0.75;   // FP64 lane-ops/s achieved (complex128)
-    double valu_rate_f32 = 78.6e12 * 0.75;
-    double mfma_f64 = 78.6e12 * 0.6;     // achieved FP64 MFMA flop/s
-    double mfma_f32 = 157.3e12 * 0.6;    // achieved FP32 MFMA flop/s
+    double hbm_bw = 5.457e+12;       // achieved transpose bandwidth, B/s (TransMM passes)
+    // Achieved bandwidth of one OShareMem pass (read + write of the state) by
+    // the run length L of its tile (the number of lowest physical qubits in the
+    // tile: 2^L consecutive amplitudes per run), complex128 and complex64.
+    // Measured by the calibration tool with zero-gate passes at 30 qubits.
+    double shm_bw_run[13] = {1.589e+12, 2.191e+12, 3.851e+12, 4.276e+12, 4.957e+12, 5.04e+12, 5.522e+12,
+                             4.973e+12, 5.308e+12, 5.398e+12, 5.381e+12, 5.263e+12, 5.017e+12};
+    double shm_bw_run_f32[13] = {1.143e+12, 1.413e+12, 2.329e+12, 3.933e+12, 4.727e+12, 5.259e+12, 5.165e+12,
+                                 5.393e+12, 4.531e+12, 5.435e+12, 5.352e+12, 5.481e+12, 5.301e+12};
+    double valu_rate = 3.793e+13;    // FP64 lane-ops/s achieved (complex128)
+    double valu_rate_f32 = 3.84e+13;
+    double mfma_f64 = 3.886e+13;     // achieved FP64 MFMA flop/s (dense kernel, K = 64)
+    double mfma_f32 = 1.124e+14;     // achieved FP32 MFMA flop/s (dense kernel, K = 128)
     double lds_per_amp_stage = 0.0;      // s per amplitude per LDS exchange (0: from bandwidth)
-    double launch_overhead = 6e-6;       // s per kernel launch
+    double launch_overhead = 2.642e-06;   // s per kernel launch
     double lds_bw = 256 * 128 * 2.4e9;   // B/s whole chip (128 B/clk/CU)
 
     static CostModel referenceV100(int numQubits);
     static CostModel mi355x();
-    bool load(const std::string& path, std::string* err);  // parameter file
+    // Parameter file: this build's "key value" lines, or the reference's
+    // <n>qubits.out tables (evaluator-preprocess format), which select the
+    // reference's evaluation rules with the file's numbers.
+    bool load(const std::string& path, std::string* err);
     bool save(const std::string& path) const;
 
     // VALU instructions per amplitude for one gate in the register kernel
     static double valuPerAmp(const Gate& g);
-    // ms, like the reference's Evaluator
-    double perfPerGate(int numQubits, int precision, const std::vector<const Gate*>& gates) const;
+    // ms, like the reference's Evaluator.  tileMask: physical qubits of the
+    // group's tile (0: a tile whose run length is 4).
+    double perfPerGate(int numQubits, int precision, const std::vector<const Gate*>& gates,
+                       qindex tileMask = 0) const;
+    // OShareMem pass bandwidth for a tile with run length L
+    double shmBandwidth(int precision, int runLength) const;
+    static int runLength(qindex tileMask);
     double perfBLAS(int numQubits, int precision, int matSize) const;
 };

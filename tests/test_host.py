@@ -139,6 +139,56 @@ def test_beam_search_reduces_30q_groups(hq):
     assert beam < len(c.schedule())
 
 
+V100_SINGLES = [225, 469, 469, 352, 350, 350, 194, 209, 209, 216, 216, 370, 367, 369]   # U1 .. RZ
+V100_CTRLS = [195, 193, 137, 224, 225, 225, 224]                                      # CNOT .. CRZ
+
+
+def test_cost_model_reference_format(hq, tmp_path):
+    """The reference's <n>qubits.out format (src/evaluator.cpp:60-103) holding
+    its own built-in V100 numbers gives the same partition as the built-in
+    tables."""
+    rows = ["1"] + [str(v) for v in V100_SINGLES] + [""] + [str(v) for v in V100_CTRLS] + [""]
+    rows += [f"{1 << i} {23.068396 if i == 6 else 0.0}" for i in range(10)] + ["", "11.367814"]
+    f = tmp_path / "28qubits.out"
+    f.write_text("\n".join(rows) + "\n")
+    spec = random_mixed(28, 300, 3, ccx=False)
+    want, wpos = _product_partition(hq, 28, spec, 4)
+    c = hq.Circuit(28)
+    for s in spec:
+        c.addGate(hq.Gate.make(s[0], -1, s[2], s[1], s[3] if len(s) > 3 else ()))
+    c.set_option("profile", 1)
+    c.set_option("mode", 4)
+    c.load_cost_model(str(f))
+    c.partition()
+    got = [(g["backend"], g["num_gates"], g["related"]) for g in c.schedule()]
+    assert got == [(g["backend"], len(g["gates"]), g["related"]) for g in want]
+    assert c.final_pos() == wpos
+    # a BLAS-heavy file moves the choice to OShareMem everywhere
+    rows2 = list(rows)
+    rows2[-1] = "1000000"
+    f.write_text("\n".join(rows2) + "\n")
+    c.load_cost_model(str(f))
+    c.partition()
+    assert all(g["backend"] == hq.HQ_BACKEND_PERGATE for g in c.schedule())
+
+
+def test_cost_model_key_value_format(hq, tmp_path):
+    f = tmp_path / "m.params"
+    kv = ["# comment", "valu_rate 3.8e13", "valu_rate_f32 3.8e13"] + \
+         [f"shm_bw_run_{L} {1e12 * (1 + L)}" for L in range(13)] + [f"shm_bw_run_f32_{L} 4e12" for L in range(13)]
+    f.write_text("\n".join(kv) + "\n")
+    c = hq.Circuit(24).add_random_layers(10, 1)
+    c.load_cost_model(str(f))
+    c.partition()
+    assert len(c.schedule()) > 0
+    f.write_text("nonsense_key 1\n")
+    with pytest.raises(hq.HQError):
+        c.load_cost_model(str(f))
+    f.write_text("7\n1 2\n")
+    with pytest.raises(hq.HQError):
+        c.load_cost_model(str(f))
+
+
 def test_ccx_second_control_is_a_dependency(hq):
     """H(2) before CCX(0, 2 -> 1) must stay before it (the reference's
     getGroupOpt ignores controlQubit2 and may reorder them)."""
