@@ -91,6 +91,28 @@ def hea_layers(n, layers, seed):
     return out
 
 
+_DAGGER = {S: SDG, SDG: S, T: TDG, TDG: T}
+_NEGATE = {RX, RY, RZ, U1, CRX, CRY, CRZ, CU1}
+
+
+def inverse(spec):
+    """The inverse circuit: gates in reverse order, each replaced by its
+    adjoint (self-inverse gates kept, S/T swapped with their daggers,
+    rotation angles negated)."""
+    out = []
+    for s in reversed(spec):
+        t = s[0]
+        if t in _DAGGER:
+            out.append((_DAGGER[t],) + tuple(s[1:]))
+        elif t in _NEGATE:
+            out.append((t, s[1], s[2], tuple(-x for x in s[3])) + tuple(s[4:]))
+        elif t in (H, X, Y, Z, CNOT, CY, CZ, CCX, ID):
+            out.append(s)
+        else:
+            raise ValueError(f"no inverse rule for gate type {t}")
+    return out
+
+
 def to_oracle(spec):
     """Tuples -> oracle gate array."""
     arr = np.zeros(len(spec), dtype=oracle.GATE_DTYPE)

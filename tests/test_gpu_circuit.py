@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import random_layers, random_mixed, rel_err, to_oracle
+from helpers import inverse, random_layers, random_mixed, rel_err, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -119,3 +119,28 @@ def test_c64_fidelity(gpu):
     want = oracle.run_plain(to_oracle(random_layers(n, 10, 11)), n)
     fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
     assert 1 - fid <= 1e-5
+
+
+@pytest.mark.parametrize("n,layers,prec", [(30, 30, "c128"), (33, 24, "c64")])
+def test_round_trip_at_full_size(gpu, n, layers, prec):
+    """The headline circuits at their full sizes, followed by their inverse,
+    must return |0...0>: a size-independent check of the whole path (beam
+    partition, specialised kernels, scalar factoring, fused first group)."""
+    spec = random_layers(n, layers, 0)
+    c = _circuit(gpu, n, spec + inverse(spec))
+    p = gpu.HQ_C128 if prec == "c128" else gpu.HQ_C64
+    c.set_option("precision", p)
+    c.run(copy_back=False, destroy=False)
+    st = c.state_view(p)
+    norm = st.norm2()
+    pos = c.final_pos()
+    assert sorted(pos) == list(range(n))
+    a0 = st.get_amp(0)
+    fid = abs(a0) ** 2
+    if prec == "c128":
+        assert abs(norm - 1) < 1e-12
+        assert 1 - fid < 1e-12
+    else:
+        assert abs(norm - 1) < 2e-5
+        assert 1 - fid / norm < 1e-6
+    c.close()

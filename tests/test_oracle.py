@@ -358,3 +358,21 @@ def test_parse_qasm_dialect(ora):
 def test_parse_qasm_rejects(ora, bad):
     with pytest.raises(ValueError):
         oracle.parse_qasm(bad)
+
+
+def test_inverse_helper_is_the_adjoint():
+    # U then U^dagger is the identity on a random state: pins the adjoint rules
+    # of helpers.inverse used by the full-size round-trip GPU tests
+    from helpers import inverse, random_layers
+    n = 8
+    extra = [(S, 1, -1, ()), (SDG, 2, -1, ()), (TDG, 3, -1, ()), (X, 0, -1, ()), (Y, 4, -1, ()), (Z, 5, -1, ()),
+             (CZ, 1, 0, ()), (CRX, 2, 1, (0.3,)), (CRY, 3, 2, (1.1,)), (CRZ, 4, 3, (-0.7,)), (CU1, 5, 4, (2.2,)),
+             (CCX, 6, 0, (), 1)]
+    spec = random_layers(n, 6, 3) + extra + random_layers(n, 2, 9)
+    rng = np.random.default_rng(1)
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    out = oracle.run_plain(to_oracle(spec + inverse(spec)), n, psi0)
+    assert rel_err(out, psi0) <= 1e-13
+    half = oracle.run_plain(to_oracle(spec), n, psi0)
+    assert rel_err(half, psi0) > 0.1
