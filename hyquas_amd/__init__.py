@@ -124,6 +124,8 @@ def _declare(L):
         "hq_circuit_get_gates": (S, [c_void_p, POINTER(HQGate), c_int, POINTER(c_int)]),
         "hq_circuit_add_random_layers": (S, [c_void_p, c_int, c_uint64]),
         "hq_circuit_add_hea_layers": (S, [c_void_p, c_int, c_uint64]),
+        "hq_circuit_save_schedule": (S, [c_void_p, c_char_p]),
+        "hq_circuit_load_schedule": (S, [c_void_p, c_char_p]),
         "hq_engine_create": (S, [c_int, c_uint64, POINTER(c_void_p)]),
         "hq_engine_destroy": (S, [c_void_p]),
         "hq_engine_set_option": (S, [c_void_p, c_char_p, c_int64]),
@@ -146,6 +148,7 @@ def _declare(L):
         "hq_batch_info": (S, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_uint64)]),
         "hq_batch_destroy": (S, [c_void_p]),
         "hq_state_topk": (S, [c_void_p, c_int, POINTER(c_uint64), POINTER(c_double)]),
+        "hq_state_sample": (S, [c_void_p, c_int, c_uint64, POINTER(c_uint64)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -448,6 +451,12 @@ class State:
         _check(lib().hq_state_topk(self.handle, k, idx, prob))
         return list(idx[:]), list(prob[:])
 
+    def sample(self, shots: int, seed: int = 0) -> np.ndarray:
+        """`shots` measurement samples (physical basis indices) drawn with the portable generator."""
+        out = np.zeros(max(1, shots), dtype=np.uint64)
+        _check(lib().hq_state_sample(self.handle, shots, seed, out.ctypes.data_as(POINTER(c_uint64))))
+        return out[:shots]
+
     def profile(self, enable: bool):
         _check(lib().hq_profile_enable(self.handle, 1 if enable else 0))
 
@@ -526,6 +535,15 @@ class Circuit:
 
     def set_option(self, key: str, value: int):
         _check(lib().hq_circuit_set_option(self.handle, key.encode(), int(value)))
+        return self
+
+    def save_schedule(self, path: str):
+        """Write the partitioned schedule (compile-once cache)."""
+        _check(lib().hq_circuit_save_schedule(self.handle, path.encode()))
+
+    def load_schedule(self, path: str):
+        """Use a saved schedule instead of partitioning; builds the device programs."""
+        _check(lib().hq_circuit_load_schedule(self.handle, path.encode()))
         return self
 
     def load_cost_model(self, path: str):

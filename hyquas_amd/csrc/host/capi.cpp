@@ -1,5 +1,6 @@
 // capi.cpp -- C ABI of the Circuit layer (include/hyquas_amd.h, Circuit part).
 
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -206,6 +207,35 @@ hq_status hq_circuit_partition(hq_circuit c)
     HQ_CARG(c, "hq_circuit_partition: null circuit");
     hq_status rc;
     HQ_GUARD(rc = c->c->partitionChecked());
+    return rc;
+}
+
+hq_status hq_circuit_save_schedule(hq_circuit c, const char* path)
+{
+    HQ_CARG(c && path, "hq_circuit_save_schedule: null argument");
+    std::vector<unsigned char> bytes;
+    hq_status rc;
+    HQ_GUARD(rc = c->c->saveSchedule(&bytes));
+    if (rc != HQ_OK) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) { hq::set_error(std::string("hq_circuit_save_schedule: cannot open ") + path); return HQ_ERR_ARG; }
+    const bool ok = fwrite(bytes.data(), 1, bytes.size(), f) == bytes.size();
+    if (fclose(f) != 0 || !ok) { hq::set_error("hq_circuit_save_schedule: write failed"); return HQ_ERR_ARG; }
+    return HQ_OK;
+}
+
+hq_status hq_circuit_load_schedule(hq_circuit c, const char* path)
+{
+    HQ_CARG(c && path, "hq_circuit_load_schedule: null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) { hq::set_error(std::string("hq_circuit_load_schedule: cannot open ") + path); return HQ_ERR_ARG; }
+    std::vector<unsigned char> bytes;
+    unsigned char buf[1 << 16];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof(buf), f)) > 0) bytes.insert(bytes.end(), buf, buf + k);
+    fclose(f);
+    hq_status rc;
+    HQ_GUARD(rc = c->c->loadSchedule(bytes.data(), bytes.size()));
     return rc;
 }
 

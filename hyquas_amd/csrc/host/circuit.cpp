@@ -2,6 +2,8 @@
 
 #include "circuit.h"
 
+#include <cstring>
+
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
@@ -117,6 +119,87 @@ void Circuit::compile()
 {
     hq_status rc = compileChecked();
     if (rc != HQ_OK) throw hq::Error(rc, hq_last_error());
+}
+
+static const char kSchedMagic[8] = {'H', 'Q', 'S', 'C', 'H', 'E', 'D', '1'};
+
+hq_status Circuit::saveSchedule(std::vector<unsigned char>* out) const
+{
+    if (params.mode == HQ_MODE_SERIAL || schedule.localGroups.empty()) {
+        hq::set_error("saveSchedule: partition or compile the circuit first (not in serial mode)");
+        return HQ_ERR_STATE;
+    }
+    std::vector<unsigned char> body = schedule.serialize();
+    out->assign(kSchedMagic, kSchedMagic + 8);
+    uint32_t hdr[2] = {(uint32_t)numQubits, 0};
+    out->insert(out->end(), reinterpret_cast<unsigned char*>(hdr), reinterpret_cast<unsigned char*>(hdr) + 8);
+    out->insert(out->end(), body.begin(), body.end());
+    return HQ_OK;
+}
+
+hq_status Circuit::loadSchedule(const unsigned char* data, size_t len)
+{
+    if (params.mode == HQ_MODE_SERIAL) {
+        hq::set_error("loadSchedule: serial mode runs without a schedule");
+        return HQ_ERR_STATE;
+    }
+    if (len < 16 || memcmp(data, kSchedMagic, 8) != 0) {
+        hq::set_error("loadSchedule: not a schedule file");
+        return HQ_ERR_PARSE;
+    }
+    uint32_t hdr[2];
+    memcpy(hdr, data + 8, 8);
+    if ((int)hdr[0] != numQubits) {
+        hq::set_error("loadSchedule: the schedule is for " + std::to_string(hdr[0]) + " qubits");
+        return HQ_ERR_ARG;
+    }
+    Schedule s;
+    size_t cur = 16;
+    try {
+        s = Schedule::deserialize(data, len, cur);
+    } catch (const std::exception& e) {
+        hq::set_error(std::string("loadSchedule: ") + e.what());
+        return HQ_ERR_PARSE;
+    }
+    if (cur != len) {
+        hq::set_error("loadSchedule: trailing bytes");
+        return HQ_ERR_PARSE;
+    }
+    // every gate of the circuit exactly once, identical to the circuit's
+    std::vector<char> seen(gates.size(), 0);
+    for (auto& lg : s.localGroups) {
+        for (auto& gg : lg.fullGroups) {
+            if (gg.gateIndex.size() != gg.gates.size()) { hq::set_error("loadSchedule: bad group"); return HQ_ERR_PARSE; }
+            for (size_t i = 0; i < gg.gates.size(); i++) {
+                const int idx = gg.gateIndex[i];
+                if (idx < 0 || idx >= (int)gates.size() || seen[idx]) {
+                    hq::set_error("loadSchedule: gate index " + std::to_string(idx) + " missing or repeated");
+                    return HQ_ERR_ARG;
+                }
+                seen[idx] = 1;
+                const Gate& a = gates[idx];
+                const Gate& b = gg.gates[i];
+                if (a.type != b.type || a.targetQubit != b.targetQubit || a.controlQubit != b.controlQubit ||
+                    a.controlQubit2 != b.controlQubit2 || memcmp(a.mat, b.mat, sizeof(a.mat)) != 0) {
+                    hq::set_error("loadSchedule: gate " + std::to_string(idx) + " differs from the circuit's");
+                    return HQ_ERR_ARG;
+                }
+            }
+        }
+    }
+    for (size_t i = 0; i < seen.size(); i++)
+        if (!seen[i]) { hq::set_error("loadSchedule: gate " + std::to_string(i) + " is not scheduled"); return HQ_ERR_ARG; }
+    schedule = std::move(s);
+    compiled = false;
+    auto t0 = std::chrono::steady_clock::now();
+    exec.localSize = params.localSize;
+    hq_status rc = ensureState();
+    if (rc != HQ_OK) return rc;
+    rc = Executor::prepare(schedule, numQubits, state, exec);
+    if (rc != HQ_OK) return rc;
+    compiled = true;
+    compileMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return HQ_OK;
 }
 
 hq_status Circuit::runChecked(bool copy_back, bool destroy, long long* us)

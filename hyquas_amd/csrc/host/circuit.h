@@ -51,6 +51,11 @@ public:
     hq_status partitionChecked();   // host-only: schedule without device programs
     hq_status compileChecked();
     hq_status runChecked(bool copy_back, bool destroy, long long* us);
+    // Compile-once cache: the partitioned schedule as bytes (16-byte header +
+    // the reference's layout, src/schedule.cpp:156-325); loading checks every
+    // gate against this circuit and builds the device programs.
+    hq_status saveSchedule(std::vector<unsigned char>* out) const;
+    hq_status loadSchedule(const unsigned char* data, size_t len);
     std::string stateText(bool showSchedule) const;
 
     // compile-time options of the reference, runtime here

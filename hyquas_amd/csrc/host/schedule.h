@@ -62,4 +62,8 @@ struct Schedule {
     bool initFused = false;                // the first group's program builds |0...0> itself
     void dump(int numQubits) const;        // SHOW_SCHEDULE print (src/schedule.cpp:66-154)
     void clear();
+    // Byte form (src/schedule.cpp:156-325); deserialize throws on malformed input
+    // and advances cur.
+    std::vector<unsigned char> serialize() const;
+    static Schedule deserialize(const unsigned char* arr, size_t len, size_t& cur);
 };

@@ -254,6 +254,14 @@ hq_status hq_circuit_compile(hq_circuit c);
  * host work, no device needed; hq_circuit_schedule* then describe it. */
 hq_status hq_circuit_partition(hq_circuit c);
 
+/* Compile-once cache of the partition: the schedule in the reference's byte
+ * layout (Schedule::serialize, src/schedule.cpp:156-325, src/gate.cpp:558-589)
+ * after a 16-byte header ("HQSCHED1", qubit count).  Save needs a partitioned
+ * or compiled circuit; load checks every gate against c (type, qubits,
+ * matrix, each index once) and builds the device programs, as compile does. */
+hq_status hq_circuit_save_schedule(hq_circuit c, const char *path);
+hq_status hq_circuit_load_schedule(hq_circuit c, const char *path);
+
 /* Circuit::run (src/circuit.cpp:16-71): allocate (first call) and reset the
  * state, run every group, wait.  *us receives the wall time of the group
  * loop in microseconds (the reference's "Time Cost").  copy_back keeps a
@@ -373,6 +381,13 @@ hq_status hq_batch_destroy(hq_batch b);
 /* The k (<= 16) most probable physical basis states of s, probability
  * descending, ties by smaller index; FP64.  Synchronous. */
 hq_status hq_state_topk(hq_state s, int k, uint64_t *idx, double *prob);
+
+/* `shots` measurement samples of s (physical basis indices, in draw order):
+ * shot k is the smallest i with sum_{j<=i} |a_j|^2 > u_k * sum_j |a_j|^2, u_k
+ * the k-th uniform of the portable generator seeded with `seed`.  No
+ * counterpart in the reference (it reads P(q = 0), src/kernelSimple.cu:497-516).
+ * Synchronous. */
+hq_status hq_state_sample(hq_state s, int shots, uint64_t seed, uint64_t *out);
 
 #ifdef __cplusplus
 }

@@ -144,3 +144,27 @@ def test_round_trip_at_full_size(gpu, n, layers, prec):
         assert abs(norm - 1) < 2e-5
         assert 1 - fid / norm < 1e-6
     c.close()
+
+
+@pytest.mark.parametrize("n,mode", [(16, "mix"), (14, "blas"), (24, "mix")])
+def test_schedule_cache_round_trip(gpu, tmp_path, n, mode):
+    """A saved schedule loaded into a fresh circuit runs to the same amplitudes
+    (bit for bit) as the circuit that partitioned it."""
+    spec = random_layers(n, 8, 21)
+    m = dict(mix=4, blas=3)[mode]
+    a = _circuit(gpu, n, spec)
+    a.set_option("mode", m)
+    a.compile()
+    a.run(copy_back=False, destroy=False)
+    want = a.logical_state()
+    path = str(tmp_path / "s.bin")
+    a.save_schedule(path)
+    b = _circuit(gpu, n, spec)
+    b.set_option("mode", m)
+    b.load_schedule(path)
+    b.run(copy_back=False, destroy=False)
+    got = b.logical_state()
+    assert np.array_equal(got, want)
+    assert b.schedule_order() == a.schedule_order()
+    ref = oracle.run_plain(to_oracle(spec), n)
+    assert rel_err(got, ref) <= 1e-10

@@ -176,3 +176,34 @@ def test_state_topk_ties(gpu):
     idx, prob = s.topk(16)
     assert idx == list(range(16))
     s.close()
+
+
+@pytest.mark.parametrize("n,prec", [(9, "c128"), (20, "c128"), (20, "c64"), (13, "c64")])
+def test_state_sample_matches_inverse_cdf(gpu, n, prec):
+    from helpers import Rng
+    hq = gpu
+    p = hq.HQ_C128 if prec == "c128" else hq.HQ_C64
+    s = hq.State(n, p)
+    rng = np.random.default_rng(n)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi = (psi / np.linalg.norm(psi)).astype(np.complex128 if p == hq.HQ_C128 else np.complex64)
+    s.upload(psi)
+    shots = 3000
+    got = s.sample(shots, seed=77)
+    prob = np.abs(psi.astype(np.complex128)) ** 2
+    cdf = np.cumsum(prob)
+    r = Rng(77)
+    u = np.array([r.uniform() for _ in range(shots)]) * cdf[-1]
+    want = np.searchsorted(cdf, u, side="right")
+    assert np.array_equal(got.astype(np.int64), want)
+    s.close()
+
+
+def test_state_sample_basis_state(gpu):
+    hq = gpu
+    s = hq.State(16)
+    psi = np.zeros(1 << 16, complex)
+    psi[12345] = 1j
+    s.upload(psi)
+    assert set(s.sample(100, 3).tolist()) == {12345}
+    s.close()

@@ -236,3 +236,106 @@ def test_no_device_error_path(hq):
     c = hq.Circuit(10).add_random_layers(2, 0)
     with pytest.raises(hq.HQError):
         c.compile()
+
+
+def _parse_schedule(b):
+    """Independent reader of the schedule byte layout (reference
+    Schedule/LocalGroup/GateGroup/State/Gate::serialize, packed, little-endian)."""
+    import struct
+    cur = [0]
+
+    def take(fmt):
+        v = struct.unpack_from("<" + fmt, b, cur[0])
+        cur[0] += struct.calcsize("<" + fmt)
+        return v
+
+    def state():
+        (n,) = take("Q")
+        return list(take(f"{n}i")), list(take(f"{n}i"))
+
+    def gate():
+        gid, typ = take("ii")
+        mat = take("8d")
+        (nl,) = take("Q")
+        name = b[cur[0]:cur[0] + nl].decode()
+        assert b[cur[0] + nl] == 0
+        cur[0] += nl + 1
+        t, c1, c2 = take("iii")
+        return dict(id=gid, type=typ, mat=mat, name=name, target=t, control=c1, control2=c2)
+
+    def group():
+        ng, rel, nperm, matq, backend = take("QqQii")
+        st = state()
+        gs = [gate() for _ in range(ng)]
+        perm = list(take(f"{nperm}i"))
+        return dict(related=rel, mat_qubit=matq, backend=backend, state=st, gates=gs, perm=perm)
+
+    assert b[:8] == b"HQSCHED1"
+    nq, zero = struct.unpack_from("<II", b, 8)
+    cur[0] = 16
+    (nlg,) = take("Q")
+    final = state()
+    lgs = []
+    for _ in range(nlg):
+        a2a, na, np_, no, nf, rel = take("iQQQQq")
+        take(f"{na}i"), take(f"{np_}i")
+        st = state()
+        assert no == 0
+        lgs.append(dict(related=rel, state=st, groups=[group() for _ in range(nf)]))
+    assert cur[0] == len(b)
+    return nq, final, lgs
+
+
+def test_schedule_bytes_follow_the_reference_layout(hq, tmp_path):
+    n = 10
+    c = hq.Circuit(n).add_random_layers(3, 5)
+    c.addGate(hq.Gate.CCX(0, 1, 2))
+    c.set_option("mode", 3)          # TransMM groups: permutations and layouts in the bytes
+    c.set_option("blas_mat", 4)
+    c.partition()
+    path = str(tmp_path / "s.bin")
+    c.save_schedule(path)
+    nq, final, lgs = _parse_schedule(open(path, "rb").read())
+    assert nq == n and len(lgs) == 1
+    sched = c.schedule()
+    groups = lgs[0]["groups"]
+    assert [g["backend"] for g in groups] == [s["backend"] for s in sched]
+    assert [g["related"] for g in groups] == [s["related"] for s in sched]
+    assert [g["mat_qubit"] for g in groups] == [s["mat_qubit"] for s in sched]
+    assert [gt["id"] for g in groups for gt in g["gates"]] == c.schedule_order()
+    assert final[0] == c.final_pos()
+    arr = c.gate_array()
+    for g in groups:
+        assert sorted(g["perm"]) == (list(range(n)) if g["perm"] else [])
+        for gt in g["gates"]:
+            ref = arr[gt["id"]]
+            assert (gt["type"], gt["target"], gt["control"], gt["control2"]) == \
+                   (ref["type"], ref["target"], ref["control"], ref["control2"])
+            assert list(gt["mat"]) == list(ref["mat"])
+            assert gt["name"]
+
+
+def test_schedule_load_validates(hq, tmp_path):
+    n = 12
+    c = hq.Circuit(n).add_random_layers(4, 1)
+    c.partition()
+    path = str(tmp_path / "s.bin")
+    c.save_schedule(path)
+    other = hq.Circuit(n).add_random_layers(4, 2)          # different angles and qubits
+    with pytest.raises(hq.HQError) as e:
+        other.load_schedule(path)
+    assert e.value.status == hq.HQ_ERR_ARG
+    wrong_n = hq.Circuit(n + 1).add_random_layers(4, 1)
+    with pytest.raises(hq.HQError):
+        wrong_n.load_schedule(path)
+    data = open(path, "rb").read()
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(data[:len(data) // 2])
+    same = hq.Circuit(n).add_random_layers(4, 1)
+    with pytest.raises(hq.HQError) as e:
+        same.load_schedule(str(bad))
+    assert e.value.status == hq.HQ_ERR_PARSE
+    if hq.device_count() == 0:
+        with pytest.raises(hq.HQError) as e:
+            same.load_schedule(path)      # valid; building programs needs the GPU
+        assert e.value.status == hq.HQ_ERR_NO_DEVICE
