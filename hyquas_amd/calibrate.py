@@ -48,11 +48,11 @@ def _time_program(st, prog, reps=5):
 
 
 def run_length_table(n, precision, samples=4, seed=0):
-    """Bandwidth (B/s) of a zero-gate OShareMem pass by the run length L of its
-    tile, L = 0..12: the tile is the lowest L qubits plus 12 - L qubits drawn
-    at random above qubit L (the median of `samples` draws).  The pass runs in
-    the interpreter kernel: a specialised kernel without gates loads and stores
-    the same values, which the compiler removes."""
+    """Bandwidth (B/s) of an OShareMem pass by the run length L of its tile,
+    L = 0..12: the tile is the lowest L qubits plus 12 - L qubits drawn at
+    random above qubit L (median of `samples` draws).  The pass is a
+    specialised kernel holding one diagonal gate (a kernel without gates loads
+    and stores the same values, which the compiler removes)."""
     st = hq.State(n, precision)
     st.profile(True)
     b = 16 if precision == hq.HQ_C128 else 8
@@ -65,7 +65,9 @@ def run_length_table(n, precision, samples=4, seed=0):
             mask = (1 << L) - 1
             for q in rest:
                 mask |= 1 << int(q)
-            p = st.program_sharemem(hq.gates_to_array([]), mask)
+            low = min(q for q in range(n) if (mask >> q) & 1)
+            p = st.program_sharemem(hq.gates_to_array([hq.Gate.RZ(low, 0.25)]), mask)
+            p.specialize()
             ts.append(_time_program(st, p, reps=3))
             p.close()
         ms = float(np.median(ts))

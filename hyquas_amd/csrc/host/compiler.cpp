@@ -411,21 +411,21 @@ Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerPa
     const bool search = p.coalesceAuto && !p.cost.reference && (p.mode == HQ_MODE_GROUP || p.mode == HQ_MODE_MIX);
     if (!search) return compileOne(n, gates, p, &cost);
     // the coalesced (always-in-tile) low qubits trade run length against free
-    // tile slots: keep the partition with the lowest predicted time
-    Schedule best;
-    double bestCost = 0;
-    bool have = false;
+    // tile slots: another coalesce replaces the default only when its
+    // predicted time is lower by more than the model's error (3 %; passes of
+    // one partition vary by 1-2 % between runs and boxes)
+    Schedule best = compileOne(n, gates, p, &cost);
+    double bestCost = cost;
+    const double base = cost;
     for (int c = 2; c <= 5; c++) {
-        if (c >= std::min(p.localSize, n)) break;
+        if (c == p.coalesce || c >= std::min(p.localSize, n)) continue;
         CompilerParams q = p;
         q.coalesce = c;
         Schedule s = compileOne(n, gates, q, &cost);
-        if (!have || cost < bestCost) {
+        if (cost < bestCost && cost < 0.97 * base) {
             best = std::move(s);
             bestCost = cost;
-            have = true;
         }
     }
-    if (!have) return compileOne(n, gates, p, &cost);
     return best;
 }

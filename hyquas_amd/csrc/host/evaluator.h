@@ -29,11 +29,12 @@ struct CostModel {
     // Achieved bandwidth of one OShareMem pass (read + write of the state) by
     // the run length L of its tile (the number of lowest physical qubits in the
     // tile: 2^L consecutive amplitudes per run), complex128 and complex64.
-    // Measured by the calibration tool with zero-gate passes at 30 qubits.
-    double shm_bw_run[13] = {1.589e+12, 2.191e+12, 3.851e+12, 4.276e+12, 4.957e+12, 5.04e+12, 5.522e+12,
-                             4.973e+12, 5.308e+12, 5.398e+12, 5.381e+12, 5.263e+12, 5.017e+12};
-    double shm_bw_run_f32[13] = {1.143e+12, 1.413e+12, 2.329e+12, 3.933e+12, 4.727e+12, 5.259e+12, 5.165e+12,
-                                 5.393e+12, 4.531e+12, 5.435e+12, 5.352e+12, 5.481e+12, 5.301e+12};
+    // Measured by the calibration tool (specialised kernels holding one
+    // diagonal gate, random tiles) at 30 qubits.
+    double shm_bw_run[13] = {1.586e+12, 1.651e+12, 3.915e+12, 4.622e+12, 4.923e+12, 4.822e+12, 5.214e+12,
+                             5.191e+12, 5.338e+12, 5.279e+12, 5.17e+12, 5.203e+12, 5.197e+12};
+    double shm_bw_run_f32[13] = {1.167e+12, 1.401e+12, 2.328e+12, 3.522e+12, 4.833e+12, 5.167e+12, 5.112e+12,
+                                 5.278e+12, 4.689e+12, 5.371e+12, 5.228e+12, 5.34e+12, 5.267e+12};
     double valu_rate = 3.793e+13;    // FP64 lane-ops/s achieved (complex128)
     double valu_rate_f32 = 3.84e+13;
     double mfma_f64 = 3.886e+13;     // achieved FP64 MFMA flop/s (dense kernel, K = 64)
