@@ -415,17 +415,19 @@ def main_precision(a, hq):
             c.set_option("blas_mat", mat)
         c.compile()
         sec, launches = _timed_run(hq, c, 1, 1 if mode == hq.HQ_MODE_GROUP else 0)
+        # squared norm from the device's FP64 (Kahan) reduction
         if mode == hq.HQ_MODE_GROUP:
             ov = ref_state.inner(c.state_view(hq.HQ_C64))
+            n64 = c.state_view(hq.HQ_C64).norm2()
             psi = c.state_view(hq.HQ_C64).download()
         else:
             psi = c.logical_state(hq.HQ_C64)
             st = hq.State(n, hq.HQ_C64)
             st.upload(psi)
             ov = ref_state.inner(st)
+            n64 = st.norm2()
             st.close()
         d = float(np.max(np.abs(psi.astype(np.complex128) - psi_ref)))
-        n64 = float(np.vdot(psi, psi).real)
         dense = [ms for k, ms in launches if k == hq.KIND_DENSE]
         tr = [ms for k, ms in launches if k == hq.KIND_TRANSPOSE]
         # raw: |<ref|psi>|^2 with psi as computed; normalized: psi scaled to unit norm first

@@ -33,7 +33,8 @@ import numpy as np
 
 import hyquas_amd as hq
 
-TILE = 12
+TILE = 12        # VALU-rate group tile
+RUN_TILE = 13    # tile of the run-length passes: the specialised kernels' default
 
 
 def _time_program(st, prog, reps=5):
@@ -47,21 +48,21 @@ def _time_program(st, prog, reps=5):
     return t[len(t) // 2]
 
 
-def run_length_table(n, precision, samples=4, seed=0):
+def run_length_table(n, precision, samples=4, seed=0, tile=RUN_TILE):
     """Bandwidth (B/s) of an OShareMem pass by the run length L of its tile,
-    L = 0..12: the tile is the lowest L qubits plus 12 - L qubits drawn at
-    random above qubit L (median of `samples` draws).  The pass is a
-    specialised kernel holding one diagonal gate (a kernel without gates loads
-    and stores the same values, which the compiler removes)."""
+    L = 0..tile (capped at 12): the tile is the lowest L qubits plus tile - L
+    qubits drawn at random above qubit L (median of `samples` draws).  The pass
+    is a specialised kernel holding one diagonal gate (a kernel without gates
+    loads and stores the same values, which the compiler removes)."""
     st = hq.State(n, precision)
     st.profile(True)
     b = 16 if precision == hq.HQ_C128 else 8
     rng = np.random.default_rng(seed)
     out = []
-    for L in range(TILE + 1):
+    for L in range(13):
         ts = []
-        for _ in range(samples if L < TILE else 1):
-            rest = rng.choice(np.arange(L + 1, n), TILE - L, replace=False) if L < TILE else []
+        for _ in range(samples if L < tile else 1):
+            rest = rng.choice(np.arange(L + 1, n), tile - L, replace=False) if L < tile else []
             mask = (1 << L) - 1
             for q in rest:
                 mask |= 1 << int(q)

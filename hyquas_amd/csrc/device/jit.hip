@@ -22,6 +22,7 @@
 #include <fstream>
 #include <mutex>
 #include <sstream>
+#include <stdexcept>
 #include <sys/stat.h>
 #include <thread>
 #include <unordered_map>
@@ -72,7 +73,9 @@ struct Emitter {
     bool f32;
     bool nt_ld, nt_st;       // non-temporal HBM loads / stores
     int bmap;                // block index -> tile: 0 natural, 1 reversed bits, 2 contiguous per XCD
-    std::vector<int> name;   // register j is held in variable r<name[j]>
+    std::vector<int> name;   // register j is held in variable <gen><name[j]>
+    char gen = 'r';          // variable set of the current registers (r or q)
+    bool split;              // tile above 64 KiB: two-phase LDS exchanges over half the tile
 
     Emitter(const ShmPlan& p, std::vector<double>* c) : P(p), coef(c)
     {
@@ -80,6 +83,7 @@ struct Emitter {
         NT = 1 << p.tb;
         K = p.k;
         f32 = p.precision != HQ_C128;
+        split = ((size_t)1 << K) * (f32 ? 8 : 16) > 64 * 1024;
         // non-temporal accesses measured slower for these kernels on MI355X
         // (7.95 vs 7.17 ms per 30-qubit pass); HQ_JIT_NT=1 turns them on
         const char* env = getenv("HQ_JIT_NT");
@@ -93,7 +97,52 @@ struct Emitter {
         for (int j = 0; j < NR; j++) name[j] = j;
     }
 
-    std::string r(int j) const { return "r" + std::to_string(name[j]); }
+    std::string r(int j) const { return std::string(1, gen) + std::to_string(name[j]); }
+
+    // LDS index of tile index x with tile bit b removed (split exchanges)
+    uint32_t swz_split(uint32_t x, int b) const
+    {
+        const uint32_t lo = x & ((1u << b) - 1), hi = (x >> (b + 1)) << b;
+        return swz_of(lo | hi, P.hdr, K - 1);
+    }
+
+    void lane_lds_split(const ShmStage& st, int b, const char* var)
+    {
+        o << "    unsigned " << var << " = 0;\n";
+        for (int i = 0; i < P.tb; i++) {
+            uint32_t sw = swz_split(1u << st.thr_bits[i], b);
+            o << "    " << var << " ^= (0u - ((t >> " << i << ") & 1u)) & " << hex32(sw) << ";\n";
+        }
+    }
+
+    // exchange from stage A to stage B in two halves split on a register bit
+    // common to both (the planner keeps one): write the half with bit = 0,
+    // read it, then the other half; new values go to the other variable set
+    void split_exchange(const ShmStage& A, const ShmStage& B)
+    {
+        int b = -1, ra = -1, rbb = -1;
+        for (int i = P.rb - 1; i >= 0 && b < 0; i--)
+            for (int k2 = 0; k2 < P.rb; k2++)
+                if (A.reg_bits[i] == B.reg_bits[k2]) { b = (int)A.reg_bits[i]; ra = i; rbb = k2; break; }
+        if (b < 0) throw std::runtime_error("split exchange without a common register bit");
+        const char ng = gen == 'r' ? 'q' : 'r';
+        o << "    {\n";
+        lane_lds_split(A, b, "sw");
+        lane_lds_split(B, b, "sr");
+        for (int ph = 0; ph < 2; ph++) {
+            if (ph == 1) o << "    __syncthreads();\n";
+            for (int j = 0; j < NR; j++)
+                if (((j >> ra) & 1) == ph)
+                    o << "    lds[sw ^ " << hex32(swz_split(reg_tile(A, j), b)) << "] = " << r(j) << ";\n";
+            o << "    __syncthreads();\n";
+            for (int j = 0; j < NR; j++)
+                if (((j >> rbb) & 1) == ph)
+                    o << "    " << ng << j << " = lds[sr ^ " << hex32(swz_split(reg_tile(B, j), b)) << "];\n";
+        }
+        o << "    }\n";
+        gen = ng;
+        for (int j = 0; j < NR; j++) name[j] = j;
+    }
 
     // physical offset (as a literal) of register index j in stage s
     uint64_t reg_off(const ShmStage& st, int j) const
@@ -285,7 +334,7 @@ struct Emitter {
             o << "__device__ __forceinline__ void st(C* p, const C c) { V2 v; v.x = c.x; v.y = c.y; "
                  "__builtin_nontemporal_store(v, (V2*)p); }\n";
         o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") void hq_group(C* __restrict__ psi, const R* __restrict__ cf)\n{\n";
-        if (nst > 1) o << "    __shared__ C lds[" << (1 << K) << "];\n";
+        if (nst > 1) o << "    __shared__ C lds[" << (1 << (split ? K - 1 : K)) << "];\n";
         o << "    const unsigned t = threadIdx.x;\n";
         if (bmap == 2)
             o << "    const unsigned long long b = (unsigned long long)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);\n";
@@ -304,6 +353,8 @@ struct Emitter {
         }
         o << "    C";
         for (int j = 0; j < NR; j++) o << (j ? ", " : " ") << "r" << j;
+        if (split)
+            for (int j = 0; j < NR; j++) o << ", q" << j;
         o << ";\n";
         {
             const ShmStage& st = P.stages[0];
@@ -328,6 +379,11 @@ struct Emitter {
                 const ShmStage& prev = P.stages[s - 1];
                 // every lane has read the previous exchange before it is overwritten
                 if (s > 1) o << "    __syncthreads();\n";
+                if (split) {
+                    split_exchange(prev, st);
+                    for (uint32_t oi = st.op_begin; oi < st.op_end; oi++) emit_op(P.ops[oi], true);
+                    continue;
+                }
                 o << "    {\n";
                 lane_lds(prev, "sw");
                 for (int j = 0; j < NR; j++)
@@ -472,10 +528,17 @@ hq_status compile_source(const std::string& src, std::vector<char>* code, std::s
 
 }  // namespace
 
+// Source of the specialised kernel; empty (and the error set) if the plan
+// cannot be emitted.
 std::string jit_source(const ShmPlan& plan, std::vector<double>* coef)
 {
-    Emitter e(plan, coef);
-    return std::string(kSrcHeader) + e.emit();
+    try {
+        Emitter e(plan, coef);
+        return std::string(kSrcHeader) + e.emit();
+    } catch (const std::exception& ex) {
+        set_error(std::string("specialised kernel: ") + ex.what());
+        return std::string();
+    }
 }
 
 static hq_status load_program(const ShmPlan& plan, const std::vector<char>& code, const std::vector<double>& coef,
@@ -501,6 +564,7 @@ hq_status jit_sharemem(const ShmPlan& plan, hq_program p)
 {
     std::vector<double> coef;
     std::string src = jit_source(plan, &coef);
+    if (src.empty()) return HQ_ERR_UNSUPPORTED;
     std::vector<char> code;
     std::string err;
     hq_status st = compile_source(src, &code, &err);
@@ -515,7 +579,10 @@ hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std:
     const size_t n = plans.size();
     std::vector<std::string> srcs(n);
     std::vector<std::vector<double>> coefs(n);
-    for (size_t i = 0; i < n; i++) srcs[i] = jit_source(*plans[i], &coefs[i]);
+    for (size_t i = 0; i < n; i++) {
+        srcs[i] = jit_source(*plans[i], &coefs[i]);
+        if (srcs[i].empty()) return HQ_ERR_UNSUPPORTED;
+    }
     std::vector<std::vector<char>> codes(n);
     std::vector<hq_status> sts(n, HQ_OK);
     std::vector<std::string> errs(n);
@@ -569,6 +636,7 @@ extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gat
     if (st != HQ_OK) return st;
     std::vector<double> cf;
     std::string src = hq::jit_source(plan, &cf);
+    if (src.empty()) return HQ_ERR_UNSUPPORTED;
     if (len) *len = src.size();
     if (buf && cap) {
         size_t k = std::min(src.size(), cap - 1);

@@ -227,6 +227,10 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
     std::vector<Stage> stages;
     std::vector<int> pending(low.size());
     for (size_t i = 0; i < low.size(); i++) pending[i] = (int)i;
+    // a tile above 64 KiB is exchanged in two LDS halves by the specialised
+    // kernel, split on a register bit that consecutive stages share: keep one
+    const bool split_lds = ((size_t)1 << k) * amp_bytes > 64 * 1024;
+    uint32_t prev_reg = 0;
     do {
         Stage S;
         uint32_t regset = 0;
@@ -246,12 +250,17 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
             if (ok && L.needs_reg) {
                 uint32_t tb1 = 1u << L.t_tile;
                 if (!(regset & tb1)) {
-                    if (cnt < rb) { regset |= tb1; cnt++; }
+                    const bool keep = split_lds && !stages.empty() && !((regset | tb1) & prev_reg);
+                    if (cnt < (keep ? rb - 1 : rb)) { regset |= tb1; cnt++; }
                     else ok = false;
                 }
             }
             if (ok) S.ops.push_back(idx);
             else { blk_tile |= tt; blk_block |= tbk; rest.push_back(idx); }
+        }
+        if (split_lds && !stages.empty() && !(regset & prev_reg) && cnt < rb) {
+            for (int b = k - 1; b >= 0; b--)
+                if ((prev_reg >> b) & 1) { regset |= 1u << b; cnt++; break; }
         }
         // fill the register set with the highest remaining tile bits, so the
         // lanes cover the low (contiguous) tile bits
@@ -275,6 +284,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
         for (int b : lane3) S.thr_bits.push_back(b);
         for (int b : free_bits)
             if (std::find(lane3.begin(), lane3.end(), b) == lane3.end()) S.thr_bits.push_back(b);
+        prev_reg = regset;
         stages.push_back(std::move(S));
         pending.swap(rest);
     } while (!pending.empty());

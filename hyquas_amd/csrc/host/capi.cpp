@@ -127,6 +127,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
         HQ_CARG(v == 0 || v == 1, "profile must be 0 (gfx950) or 1 (reference)");
         C.referenceProfile = v == 1;
         if (v == 1) {
+            C.localSizeAuto = false;
             C.params.localSize = 10;
             C.params.coalesce = 3;
             C.params.starts = 1;
@@ -142,6 +143,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
     } else if (k == "share_qubits") {
         HQ_CARG(v >= 7 && v <= 14, "share_qubits must be in [7, 14]");
         C.params.localSize = (int)v;
+        C.localSizeAuto = false;
     } else if (k == "coalesce") {
         HQ_CARG(v >= 0 && v <= 8, "coalesce must be in [0, 8]");
         C.params.coalesce = (int)v;

@@ -88,6 +88,7 @@ hq_status Circuit::partitionChecked()
         schedule.finalState = State(numQubits);
     } else {
         if (referenceProfile) params.cost = CostModel::referenceV100(numQubits);
+        if (localSizeAuto) params.localSize = autoLocalSize();
         exec.localSize = params.localSize;
         schedule = compileSchedule(numQubits, gates, params);
     }
@@ -119,6 +120,15 @@ void Circuit::compile()
 {
     hq_status rc = compileChecked();
     if (rc != HQ_OK) throw hq::Error(rc, hq_last_error());
+}
+
+int Circuit::autoLocalSize() const
+{
+    // specialised kernels exchange a 13-qubit tile in two LDS halves and keep
+    // two blocks per CU (fewer passes, measured faster); the interpreter keeps
+    // 12-qubit tiles (64 KiB LDS)
+    const bool jit = exec.jit == 1 || (exec.jit < 0 && numQubits >= exec.jitMinQubits);
+    return jit ? 13 : 12;
 }
 
 static const char kSchedMagic[8] = {'H', 'Q', 'S', 'C', 'H', 'E', 'D', '1'};
@@ -189,6 +199,13 @@ hq_status Circuit::loadSchedule(const unsigned char* data, size_t len)
     }
     for (size_t i = 0; i < seen.size(); i++)
         if (!seen[i]) { hq::set_error("loadSchedule: gate " + std::to_string(i) + " is not scheduled"); return HQ_ERR_ARG; }
+    // the tile must hold every group's related qubits
+    int need = 0;
+    for (auto& lg : s.localGroups)
+        for (auto& gg : lg.fullGroups)
+            if (gg.backend == Backend::PerGate)
+                need = std::max(need, __builtin_popcountll((unsigned long long)gg.relatedQubits));
+    if (localSizeAuto) params.localSize = std::max(autoLocalSize(), need);
     schedule = std::move(s);
     compiled = false;
     auto t0 = std::chrono::steady_clock::now();

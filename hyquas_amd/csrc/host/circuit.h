@@ -62,6 +62,8 @@ public:
     CompilerParams params;
     ExecParams exec;
     bool referenceProfile = false;
+    bool localSizeAuto = true;   // tile qubits from the kernel kind unless "share_qubits" is set
+    int autoLocalSize() const;
 
     const std::vector<Gate>& getGates() const { return gates; }
     const Schedule& getSchedule() const { return schedule; }

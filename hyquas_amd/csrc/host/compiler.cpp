@@ -411,13 +411,15 @@ Schedule compileSchedule(int n, const std::vector<Gate>& gates, const CompilerPa
     const bool search = p.coalesceAuto && !p.cost.reference && (p.mode == HQ_MODE_GROUP || p.mode == HQ_MODE_MIX);
     if (!search) return compileOne(n, gates, p, &cost);
     // the coalesced (always-in-tile) low qubits trade run length against free
-    // tile slots: another coalesce replaces the default only when its
+    // tile slots: the other coalesce replaces the default only when its
     // predicted time is lower by more than the model's error (3 %; passes of
     // one partition vary by 1-2 % between runs and boxes)
     Schedule best = compileOne(n, gates, p, &cost);
     double bestCost = cost;
     const double base = cost;
-    for (int c = 2; c <= 5; c++) {
+    // runs shorter than 16 amplitudes were never faster once the groups carry
+    // their gates (12- and 13-qubit tiles, 30 qubits), so 4 and 5 are tried
+    for (int c = 4; c <= 5; c++) {
         if (c == p.coalesce || c >= std::min(p.localSize, n)) continue;
         CompilerParams q = p;
         q.coalesce = c;

@@ -30,11 +30,11 @@ struct CostModel {
     // the run length L of its tile (the number of lowest physical qubits in the
     // tile: 2^L consecutive amplitudes per run), complex128 and complex64.
     // Measured by the calibration tool (specialised kernels holding one
-    // diagonal gate, random tiles) at 30 qubits.
-    double shm_bw_run[13] = {1.586e+12, 1.651e+12, 3.915e+12, 4.622e+12, 4.923e+12, 4.822e+12, 5.214e+12,
-                             5.191e+12, 5.338e+12, 5.279e+12, 5.17e+12, 5.203e+12, 5.197e+12};
-    double shm_bw_run_f32[13] = {1.167e+12, 1.401e+12, 2.328e+12, 3.522e+12, 4.833e+12, 5.167e+12, 5.112e+12,
-                                 5.278e+12, 4.689e+12, 5.371e+12, 5.228e+12, 5.34e+12, 5.267e+12};
+    // diagonal gate, random 13-qubit tiles) at 30 qubits.
+    double shm_bw_run[13] = {1.323e+12, 1.491e+12, 4.122e+12, 5.198e+12, 5.054e+12, 4.958e+12, 5.3e+12,
+                             5.369e+12, 5.401e+12, 5.23e+12, 5.398e+12, 5.227e+12, 5.063e+12};
+    double shm_bw_run_f32[13] = {1.141e+12, 1.355e+12, 2.199e+12, 3.945e+12, 4.553e+12, 4.869e+12, 4.978e+12,
+                                 5.428e+12, 4.831e+12, 5.215e+12, 5.285e+12, 5.377e+12, 5.198e+12};
     double valu_rate = 3.793e+13;    // FP64 lane-ops/s achieved (complex128)
     double valu_rate_f32 = 3.84e+13;
     double mfma_f64 = 3.886e+13;     // achieved FP64 MFMA flop/s (dense kernel, K = 64)

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 @pytest.mark.parametrize("n,k,seed,prec,rb", [(10, 9, 31, 0, 0), (11, 10, 32, 0, 3), (12, 11, 33, 1, 0),
-                                              (11, 11, 34, 0, 5), (14, 13, 35, 0, 4)])
+                                              (11, 11, 34, 0, 5), (14, 13, 35, 0, 4), (15, 13, 36, 0, 5),
+                                              (14, 13, 37, 1, 5)])
 def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb):
     rs = np.random.default_rng(seed)
     rest = rs.choice(np.arange(3, n), k - 3, replace=False)
@@ -54,3 +55,14 @@ def test_exchange_barriers(hq):
     assert stages >= 3
     assert src.count("__syncthreads") == 2 * (stages - 1) - 1
     assert exchanges > 0
+
+
+def test_split_exchange_for_large_tiles(hq):
+    # a 13-qubit complex128 tile (128 KiB) is exchanged in two 64 KiB halves
+    n, mask = 15, (1 << 13) - 1
+    g = _group_for_mask(n, mask, 150, 8)
+    src, _, (tile, rb, tb) = hq.sharemem_source(n, g, mask, 0, 5)
+    assert (tile, rb, tb) == (13, 5, 8)
+    assert "__shared__ C lds[4096]" in src and " q0 = lds[" in src
+    src64, _, _ = hq.sharemem_source(n, g, mask, 1, 5)    # complex64: 64 KiB, one exchange phase
+    assert "__shared__ C lds[8192]" in src64 and " q0 = lds[" not in src64
