@@ -99,11 +99,19 @@ struct Emitter {
 
     std::string r(int j) const { return std::string(1, gen) + std::to_string(name[j]); }
 
-    // LDS index of tile index x with tile bit b removed (split exchanges)
+    // LDS index in a split exchange: the tile's swizzled index with tile bit
+    // b removed (b is a register bit, fixed within a phase; the swizzle only
+    // moves bits 0..2, so the lanes keep their bank-group spread)
     uint32_t swz_split(uint32_t x, int b) const
     {
-        const uint32_t lo = x & ((1u << b) - 1), hi = (x >> (b + 1)) << b;
-        return swz_of(lo | hi, P.hdr, K - 1);
+        if (b < 3) {
+            // the swizzle would move bit b: compact first, then swizzle the K-1 bits
+            const uint32_t lo = x & ((1u << b) - 1), hi = (x >> (b + 1)) << b;
+            return swz_of(lo | hi, P.hdr, K - 1);
+        }
+        const uint32_t y = swz_of(x, P.hdr, K);
+        const uint32_t lo = y & ((1u << b) - 1), hi = (y >> (b + 1)) << b;
+        return lo | hi;
     }
 
     void lane_lds_split(const ShmStage& st, int b, const char* var)
@@ -333,7 +341,10 @@ struct Emitter {
         if (nt_st)
             o << "__device__ __forceinline__ void st(C* p, const C c) { V2 v; v.x = c.x; v.y = c.y; "
                  "__builtin_nontemporal_store(v, (V2*)p); }\n";
-        o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") void hq_group(C* __restrict__ psi, const R* __restrict__ cf)\n{\n";
+        // two waves per SIMD (two blocks per CU): a kernel that would need more
+        // than 256 registers spills a few instead of halving the occupancy
+        o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2)))"
+          << " void hq_group(C* __restrict__ psi, const R* __restrict__ cf)\n{\n";
         if (nst > 1) o << "    __shared__ C lds[" << (1 << (split ? K - 1 : K)) << "];\n";
         o << "    const unsigned t = threadIdx.x;\n";
         if (bmap == 2)

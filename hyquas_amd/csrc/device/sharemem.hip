@@ -263,7 +263,13 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
                 if ((prev_reg >> b) & 1) { regset |= 1u << b; cnt++; break; }
         }
         // fill the register set with the highest remaining tile bits, so the
-        // lanes cover the low (contiguous) tile bits
+        // lanes cover the low (contiguous) tile bits (HQ_SHM_FILL=1: the lowest
+        // bits above the six lane bits instead)
+        static const int fill_low = getenv("HQ_SHM_FILL") ? atoi(getenv("HQ_SHM_FILL")) : 0;
+        if (fill_low == 1) {
+            for (int b = 6; b < k && cnt < rb; b++)
+                if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
+        }
         for (int b = k - 1; b >= 0 && cnt < rb; b--)
             if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
         for (int b = 0; b < k; b++)
