@@ -254,9 +254,13 @@ public:
         while ((1ull << o) < bytes) o++;
         return o;
     }
-    bool can_ever_fit(uint64_t bytes) const
+    // blocks of these power-of-two sizes fit an empty arena: each within a
+    // chunk and their sum within the whole chunks the budget allows (buddy
+    // blocks of power-of-two sizes pack without loss)
+    bool can_ever_fit(uint64_t largest_block, uint64_t total_pow2) const
     {
-        return order_of(bytes) <= top_ && (1ull << top_) <= budget_;
+        const uint64_t chunk = 1ull << top_;
+        return order_of(largest_block) <= top_ && total_pow2 <= (budget_ / chunk) * chunk;
     }
     // false: no room within the budget now (or a real device OOM)
     bool alloc(uint64_t bytes, Block* out)
@@ -656,11 +660,13 @@ static bool job_alloc(Job* j, Arena& a, int precision)
     return true;
 }
 
-static uint64_t job_bytes(const Job& j, int precision)
+static uint64_t job_bytes(const Job& j, int precision, uint64_t* largest)
 {
     uint64_t sz[5], t = 0;
     job_sizes(j, precision, &sz[0], &sz[1], &sz[2], &sz[3], &sz[4]);
-    for (uint64_t s : sz) if (s) t += 1ull << Arena::order_of(s);
+    *largest = 0;
+    for (uint64_t s : sz)
+        if (s) { t += 1ull << Arena::order_of(s); *largest = std::max(*largest, s); }
     return t;
 }
 
@@ -734,6 +740,8 @@ struct hq_engine_s {
     std::vector<int64_t> completed;    // not yet polled
     int64_t rounds = 0;
     std::set<hq_batch_s*> batches;     // live batches (they hold arena blocks)
+    uint64_t batch_bytes = 0;          // arena bytes held by live batches
+    std::string broken;                // sticky error: a round failed half-way
 
     void start_workers()
     {
@@ -848,15 +856,24 @@ hq_status admit(hq_engine e, std::vector<Job*>* admitted)
         }
         if (status == Job::FAILED) { e->admit_q.pop_front(); e->completed.push_back(j->id); continue; }
         if (status != Job::COMPILED) break;  // submission order
-        if (!e->arena.can_ever_fit(job_bytes(*j, e->precision))) {
+        uint64_t largest = 0;
+        const uint64_t need = job_bytes(*j, e->precision, &largest);
+        bool fits = e->arena.can_ever_fit(largest, need);
+        if (fits && !job_alloc(j, e->arena, e->precision)) {
+            // nothing of this engine holds memory: the circuit will never fit
+            // (budget not reachable on the device); otherwise wait for memory
+            if (!e->running.empty() || !e->inflight.empty() || e->arena.used() > e->batch_bytes) break;
+            fits = false;
+        }
+        if (!fits) {
             std::lock_guard<std::mutex> lk(e->mu);
             j->status = Job::FAILED;
-            j->error = "engine: circuit needs more device memory than the budget allows";
+            j->error = "engine: circuit needs more device memory than the budget allows (" + std::to_string(need) +
+                       " B)";
             e->admit_q.pop_front();
             e->completed.push_back(j->id);
             continue;
         }
-        if (!job_alloc(j, e->arena, e->precision)) break;  // wait for memory
         j->t_admit = now_s() - e->epoch;
         {
             std::lock_guard<std::mutex> lk(e->mu);
@@ -946,7 +963,10 @@ struct hq_batch_s {
     int launches = 0;
     ~hq_batch_s()
     {
-        if (e) e->batches.erase(this);
+        if (e) {
+            e->batches.erase(this);
+            for (auto& j : jobs) for (auto& b : j->blocks) e->batch_bytes -= 1ull << b.order;
+        }
         if (e && e->stream) (void)hipStreamSynchronize(e->stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1065,13 +1085,14 @@ hq_status hq_engine_submit(hq_engine e, hq_circuit c, int outputs, int64_t* job)
 hq_status hq_engine_step(hq_engine e, int* pending)
 {
     HQ_CHECK(e, HQ_ERR_ARG, "hq_engine_step: null engine");
+    HQ_CHECK(e->broken.empty(), HQ_ERR_HIP, "hq_engine_step: the engine failed earlier: " + e->broken);
     hq_status rc = retire(e);
-    if (rc != HQ_OK) return rc;
+    if (rc != HQ_OK) { e->broken = get_error(); return rc; }
     std::vector<Job*> admitted;
     rc = admit(e, &admitted);
-    if (rc != HQ_OK) return rc;
+    if (rc != HQ_OK) { e->broken = get_error(); return rc; }
     rc = launch_round(e, admitted);
-    if (rc != HQ_OK) return rc;
+    if (rc != HQ_OK) { e->broken = get_error(); return rc; }
     if (pending) {
         // not yet done: queued, compiling, admitted or with results in flight
         int cnt = (int)e->admit_q.size() + (int)e->running.size();
@@ -1281,6 +1302,7 @@ hq_status hq_batch_create(hq_engine e, const hq_circuit* circuits, int count, in
     HQ_HIP(hipEventCreate(&b->ev1));
     b->rounds = (int)builds.size();
     e->batches.insert(b.get());
+    for (auto& j : b->jobs) for (auto& x : j->blocks) e->batch_bytes += 1ull << x.order;
     *out = b.release();
     return HQ_OK;
 }

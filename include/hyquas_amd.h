@@ -334,7 +334,10 @@ hq_status hq_circuit_add_hea_layers(hq_circuit c, int layers, uint64_t seed);
  * Readouts per circuit: {norm, sum_q <Z_q>, sum_{q<n-1} <Z_q Z_{q+1}>} over
  * logical qubits (always) and, with HQ_OUT_TOPK, the 16 most probable basis
  * states (probability descending, ties by smaller index).  Reductions run in a
- * fixed order, so a result does not depend on the other circuits. */
+ * fixed order, so a result does not depend on the other circuits.  One host
+ * thread drives an engine (its calls are not synchronised with each other); a
+ * circuit that cannot fit the budget even alone fails instead of waiting; a HIP
+ * error during a step is sticky (later steps return it). */
 enum { HQ_OUT_EXPECT = 1, HQ_OUT_TOPK = 2 };
 enum { HQ_JOB_QUEUED = 0, HQ_JOB_COMPILED = 1, HQ_JOB_ADMITTED = 2, HQ_JOB_DONE = 3, HQ_JOB_FAILED = 4 };
 

@@ -138,6 +138,12 @@ def test_budget_admission(gpu):
     assert abs(r["z"] - z) < 1e-9 and abs(r["zz"] - zz) < 1e-9
     assert all(e.status(j) == hq.HQ_JOB_DONE for j in jobs)
     assert st["bytes_used"] == 0
+    # blocks that each fit a chunk but not together: refused, not waited for forever
+    e2 = hq.Engine(hq.HQ_C128, 1 << 26)             # one 64 MiB chunk
+    j = e2.submit(_rl(hq, 22, 2, 0), hq.HQ_OUT_EXPECT)   # 64 MiB state + programs + partials
+    e2.drain()
+    assert e2.status(j) == hq.HQ_JOB_FAILED
+    e2.close()
 
 
 def test_c64_close_to_c128(gpu):
