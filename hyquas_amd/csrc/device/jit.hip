@@ -126,17 +126,9 @@ struct Emitter {
     // exchange from stage A to stage B in two halves split on a register bit
     // common to both (the planner keeps one): write the half with bit = 0,
     // read it, then the other half; new values go to the other variable set
-    // the stage's ops; with HQ_JIT_SCHEDB=N a scheduling barrier after every N
-    // ops bounds how far the compiler interleaves them (register pressure)
     void emit_stage_ops(const ShmStage& st)
     {
-        static const int every = getenv("HQ_JIT_SCHEDB") ? atoi(getenv("HQ_JIT_SCHEDB")) : 0;
-        int cnt = 0;
-        for (uint32_t oi = st.op_begin; oi < st.op_end; oi++) {
-            emit_op(P.ops[oi], true);
-            if (every > 0 && ++cnt % every == 0 && oi + 1 < st.op_end)
-                o << "    __builtin_amdgcn_sched_barrier(0);\n";
-        }
+        for (uint32_t oi = st.op_begin; oi < st.op_end; oi++) emit_op(P.ops[oi], true);
     }
 
     void split_exchange(const ShmStage& A, const ShmStage& B)

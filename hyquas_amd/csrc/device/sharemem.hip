@@ -31,12 +31,12 @@ namespace hq {
 
 // ---- device side ----------------------------------------------------------------------
 
-template <typename R, int RB, int TB, bool NT>
+template <typename R, int RB, int TB>
 __global__ __launch_bounds__(1 << TB) void k_sharemem(cplx<R>* __restrict__ psi, const char* __restrict__ prog,
                                                      uint32_t mode)
 {
     __shared__ cplx<R> lds[1 << (RB + TB)];
-    tile_pass<R, RB, TB, NT>(psi, prog, blockIdx.x, lds, mode, 0, nullptr);
+    tile_pass<R, RB, TB>(psi, prog, blockIdx.x, lds, mode, 0, nullptr);
 }
 
 // ---- launch table -------------------------------------------------------------------------
@@ -46,7 +46,7 @@ static hq_status launch_variant(hq_state s, hq_program p)
 {
     uint64_t blocks = (uint64_t)1 << (p->n - p->tile);
     const uint32_t mode = p->plan && p->plan->init ? TILE_INIT : 0u;
-    hipLaunchKernelGGL((k_sharemem<R, RB, TB, false>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
+    hipLaunchKernelGGL((k_sharemem<R, RB, TB>), dim3((unsigned)blocks), dim3(1 << TB), 0, s->stream,
                        (cplx<R>*)s->data, (const char*)p->d_prog, mode);
     HQ_HIP(hipGetLastError());
     return HQ_OK;
@@ -263,13 +263,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
                 if ((prev_reg >> b) & 1) { regset |= 1u << b; cnt++; break; }
         }
         // fill the register set with the highest remaining tile bits, so the
-        // lanes cover the low (contiguous) tile bits (HQ_SHM_FILL=1: the lowest
-        // bits above the six lane bits instead)
-        static const int fill_low = getenv("HQ_SHM_FILL") ? atoi(getenv("HQ_SHM_FILL")) : 0;
-        if (fill_low == 1) {
-            for (int b = 6; b < k && cnt < rb; b++)
-                if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
-        }
+        // lanes cover the low (contiguous) tile bits
         for (int b = k - 1; b >= 0 && cnt < rb; b--)
             if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
         for (int b = 0; b < k; b++)

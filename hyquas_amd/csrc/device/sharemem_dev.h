@@ -108,40 +108,9 @@ __device__ __forceinline__ void zsums(uint64_t idx, int n, double* z, double* zz
     *zz = (double)((n - 1) - 2 * (int)__popcll((idx ^ (idx >> 1)) & m));
 }
 
-// Amplitude load / store; NT marks them non-temporal (streamed once per pass,
-// not kept in the caches).
-template <typename R, bool NT>
-__device__ __forceinline__ cplx<R> ld_amp(const cplx<R>* p)
-{
-    if constexpr (NT) {
-        typedef R V2 __attribute__((ext_vector_type(2)));
-        const V2 v = __builtin_nontemporal_load((const V2*)p);
-        cplx<R> c;
-        c.x = v.x;
-        c.y = v.y;
-        return c;
-    } else {
-        return *p;
-    }
-}
-
-template <typename R, bool NT>
-__device__ __forceinline__ void st_amp(cplx<R>* p, const cplx<R>& c)
-{
-    if constexpr (NT) {
-        typedef R V2 __attribute__((ext_vector_type(2)));
-        V2 v;
-        v.x = c.x;
-        v.y = c.y;
-        __builtin_nontemporal_store(v, (V2*)p);
-    } else {
-        *p = c;
-    }
-}
-
 // One OShareMem pass of tile `blk` of the group program `prog` over `psi`.
 // `mode` is uniform over the grid; the single-state kernel passes 0.
-template <typename R, int RB, int TB, bool NT = false>
+template <typename R, int RB, int TB>
 __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char* __restrict__ prog, uint64_t blk,
                                           cplx<R>* __restrict__ lds, uint32_t mode, int n,
                                           double* __restrict__ part)
@@ -196,7 +165,7 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
             uint64_t off = 0;
 #pragma unroll
             for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
-            a[j] = ld_amp<R, NT>(src + off);
+            a[j] = src[off];
         }
     }
 
@@ -283,7 +252,7 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
             uint64_t off = 0;
 #pragma unroll
             for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
-            st_amp<R, NT>(dst + off, a[j]);
+            dst[off] = a[j];
         }
     }
 }
