@@ -173,3 +173,50 @@ def group_for_mask(n, mask, m, seed):
             s = (t, new_t) + tuple(s[2:])
         spec.append(s)
     return to_oracle(spec)
+
+
+_QASM_NAMES = {CCX: "ccx", CNOT: "cx", CY: "cy", CZ: "cz", CRX: "crx", CRY: "cry", CRZ: "crz", CU1: "cu1",
+               U1: "u1", U3: "u3", H: "h", X: "x", Y: "y", Z: "z", S: "s", SDG: "sdg", T: "t", TDG: "tdg",
+               RX: "rx", RY: "ry", RZ: "rz"}
+
+
+def to_qasm(n, spec):
+    """The circuit as text in the reference's input dialect (src/main.cpp:68-231):
+    angles as plain decimals written to round-trip (%.17g), qubits ordered
+    controls first."""
+    lines = ["OPENQASM 2.0;", 'include "qelib1.inc";', f"qreg q[{n}];"]   # no creg: the dialect rejects it
+    for s in spec:
+        t, tgt, ctl = s[0], s[1], s[2]
+        params = s[3] if len(s) > 3 else ()
+        c2 = s[4] if len(s) > 4 else -1
+        name = _QASM_NAMES[t]
+        if params:
+            name += "(" + ",".join(f"{p:.17g}" for p in params) + ")"
+        qs = ([ctl] if ctl >= 0 else []) + ([c2] if c2 >= 0 else []) + [tgt]
+        if t == CCX:
+            qs = [ctl, c2, tgt]
+        lines.append(name + " " + ",".join(f"q[{q}]" for q in qs) + ";")
+    return "\n".join(lines) + "\n"
+
+
+def qft_spec(n, x):
+    """X gates preparing |x>, then the textbook QFT without the final swaps."""
+    spec = [(X, q, -1, ()) for q in range(n) if (x >> q) & 1]
+    for j in range(n - 1, -1, -1):
+        spec.append((H, j, -1, ()))
+        for k in range(j - 1, -1, -1):
+            spec.append((CU1, j, k, (math.pi / 2 ** (j - k),)))
+    return spec
+
+
+def bv_spec(n, secret):
+    """Bernstein-Vazirani on n-1 data qubits and ancilla n-1."""
+    anc = n - 1
+    spec = [(X, anc, -1, ()), (H, anc, -1, ())] + [(H, q, -1, ()) for q in range(n - 1)]
+    spec += [(CNOT, anc, q, ()) for q in range(n - 1) if (secret >> q) & 1]
+    spec += [(H, q, -1, ()) for q in range(n - 1)]
+    return spec
+
+
+def ghz_spec(n):
+    return [(H, 0, -1, ())] + [(CNOT, q + 1, q, ()) for q in range(n - 1)]

@@ -168,3 +168,51 @@ def test_schedule_cache_round_trip(gpu, tmp_path, n, mode):
     assert b.schedule_order() == a.schedule_order()
     ref = oracle.run_plain(to_oracle(spec), n)
     assert rel_err(got, ref) <= 1e-10
+
+
+def test_known_answers_at_full_size(gpu):
+    """Circuit families of the reference's benchmark inputs (bv, qft, ghz),
+    read from qasm text in the reference dialect and run at full size with
+    the default (hybrid, specialised-kernel) path; each has a closed-form
+    answer independent of the partition."""
+    import math
+    from helpers import bv_spec, ghz_spec, qft_spec, to_qasm
+    hq = gpu
+    # Bernstein-Vazirani, 28 qubits: the data register holds the secret
+    n, secret = 28, 0b101101110010110101001011011
+    c = hq.Circuit.from_qasm(to_qasm(n, bv_spec(n, secret)))
+    c.run(copy_back=False, destroy=False)
+    st = c.state_view()
+    pos = c.final_pos()
+    hit = 0
+    for anc in (0, 1):
+        logical = secret | (anc << (n - 1))
+        phys = sum(((logical >> q) & 1) << pos[q] for q in range(n))
+        hit += abs(st.get_amp(phys)) ** 2
+    assert abs(hit - 1) < 1e-12
+    c.close()
+    # GHZ, 30 qubits
+    n = 30
+    c = hq.Circuit.from_qasm(to_qasm(n, ghz_spec(n)))
+    c.run(copy_back=False, destroy=False)
+    st = c.state_view()
+    pos = c.final_pos()
+    ones = sum(1 << pos[q] for q in range(n))
+    assert abs(st.get_amp(0) - 2 ** -0.5) < 1e-14 and abs(st.get_amp(ones) - 2 ** -0.5) < 1e-14
+    assert abs(st.norm2() - 1) < 1e-13
+    c.close()
+    # QFT of a basis state, 26 qubits: amplitude k (bit-reversed) = exp(2 pi i x k / N) / sqrt(N)
+    n, x = 26, 0b10110011100011110000101101
+    N = 1 << n
+    c = hq.Circuit.from_qasm(to_qasm(n, qft_spec(n, x)))
+    c.run(copy_back=False, destroy=False)
+    st = c.state_view()
+    pos = c.final_pos()
+    rng = np.random.default_rng(0)
+    for k in rng.integers(0, N, 40).tolist() + [0, 1, N - 1]:
+        rev = int(format(k, f"0{n}b")[::-1], 2)
+        phys = sum(((rev >> q) & 1) << pos[q] for q in range(n))
+        ph = 2 * math.pi * ((x * k) % N) / N   # exact reduction before the trigonometry
+        want = complex(math.cos(ph), math.sin(ph)) / math.sqrt(N)
+        assert abs(st.get_amp(phys) - want) < 1e-12 / math.sqrt(N) * 100
+    c.close()

@@ -339,3 +339,17 @@ def test_schedule_load_validates(hq, tmp_path):
         with pytest.raises(hq.HQError) as e:
             same.load_schedule(path)      # valid; building programs needs the GPU
         assert e.value.status == hq.HQ_ERR_NO_DEVICE
+
+
+def test_qasm_writer_round_trips_through_the_reader(hq):
+    from helpers import qft_spec, to_qasm
+    n = 12
+    spec = random_layers(n, 4, 2) + qft_spec(n, 0b101101) + [(CCX, 3, 1, (), 2)]
+    c = hq.Circuit.from_qasm(to_qasm(n, spec))
+    got = c.gate_array()
+    want = to_oracle(spec)
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert (a["type"], a["target"], a["control"], a["control2"]) == \
+               (b["type"], b["target"], b["control"], b["control2"])
+        assert np.array_equal(a["mat"], b["mat"])
