@@ -132,6 +132,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
             C.params.coalesce = 3;
             C.params.starts = 1;
             C.params.beam = 1;
+            C.beamAuto = false;
             C.params.coalesceAuto = false;
             C.params.minMat = 4;
             C.params.blasMat = 6;
@@ -169,9 +170,11 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
     } else if (k == "starts") {
         HQ_CARG(v >= 1 && v <= 64, "starts must be in [1, 64]");
         C.params.starts = (int)v;
+        C.beamAuto = false;
     } else if (k == "beam") {
         HQ_CARG(v >= 1 && v <= 64, "beam must be in [1, 64]");
         C.params.beam = (int)v;
+        C.beamAuto = false;
     } else if (k == "jit") {
         HQ_CARG(v >= -1 && v <= 1, "jit must be -1 (auto), 0 or 1");
         C.exec.jit = (int)v;

@@ -89,6 +89,12 @@ hq_status Circuit::partitionChecked()
     } else {
         if (referenceProfile) params.cost = CostModel::referenceV100(numQubits);
         if (localSizeAuto) params.localSize = autoLocalSize();
+        if (beamAuto) {
+            // a wider search costs host time once and saves a pass per group it
+            // removes on every run: widen it with the state size
+            params.beam = numQubits < 26 ? 8 : numQubits < 31 ? 16 : 64;
+            params.starts = numQubits < 26 ? 4 : numQubits < 31 ? 8 : 16;
+        }
         exec.localSize = params.localSize;
         schedule = compileSchedule(numQubits, gates, params);
     }

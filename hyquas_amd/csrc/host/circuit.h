@@ -63,6 +63,7 @@ public:
     ExecParams exec;
     bool referenceProfile = false;
     bool localSizeAuto = true;   // tile qubits from the kernel kind unless "share_qubits" is set
+    bool beamAuto = true;        // beam width from the qubit count unless "beam"/"starts" is set
     int autoLocalSize() const;
 
     const std::vector<Gate>& getGates() const { return gates; }
