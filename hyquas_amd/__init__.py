@@ -85,7 +85,7 @@ def _declare(L):
         "hq_program_destroy": (S, [c_void_p]),
         "hq_program_info": (S, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "hq_program_specialize": (S, [c_void_p]),
-        "hq_sharemem_source": (S, [c_int, c_int, POINTER(HQGate), c_int, c_uint64, c_int, c_char_p, c_size_t,
+        "hq_sharemem_source": (S, [c_int, c_int, POINTER(HQGate), c_int, c_uint64, c_int, c_int, c_char_p, c_size_t,
                                    POINTER(c_size_t), POINTER(c_double), c_size_t, POINTER(c_size_t),
                                    POINTER(c_int)]),
         "hq_program_is_specialized": (S, [c_void_p, POINTER(c_int)]),
@@ -284,17 +284,18 @@ def _gate_ptr(arr: np.ndarray):
     return arr, arr.ctypes.data_as(POINTER(HQGate))
 
 
-def sharemem_source(num_qubits: int, gates, shared_mask: int, precision: int = HQ_C128, reg_qubits: int = 0):
+def sharemem_source(num_qubits: int, gates, shared_mask: int, precision: int = HQ_C128, reg_qubits: int = 0,
+                    variant: int = 0):
     """Host-only: (source, coefficients, (tile, reg, thread qubits)) of the
     specialised kernel for one OShareMem group."""
     arr, p = _gate_ptr(gates_to_array(gates) if not isinstance(gates, np.ndarray) else gates)
     ln, nc = c_size_t(), c_size_t()
     shape = (c_int * 3)()
-    _check(lib().hq_sharemem_source(num_qubits, precision, p, len(arr), shared_mask, reg_qubits, None, 0,
+    _check(lib().hq_sharemem_source(num_qubits, precision, p, len(arr), shared_mask, reg_qubits, variant, None, 0,
                                     byref(ln), None, 0, byref(nc), shape))
     buf = ctypes.create_string_buffer(ln.value + 1)
     coef = np.zeros(max(1, nc.value), dtype=np.float64)
-    _check(lib().hq_sharemem_source(num_qubits, precision, p, len(arr), shared_mask, reg_qubits, buf,
+    _check(lib().hq_sharemem_source(num_qubits, precision, p, len(arr), shared_mask, reg_qubits, variant, buf,
                                     ln.value + 1, byref(ln), coef.ctypes.data_as(POINTER(c_double)), nc.value,
                                     byref(nc), shape))
     return buf.value.decode(), coef[: nc.value], tuple(shape)

@@ -129,6 +129,7 @@ struct ShmHeader {
 struct ShmPlan {
     int n = 0, precision = 0, k = 0, rb = 0, tb = 0;
     bool init = false;        // start from |0...0> instead of loading (first group of a run)
+    bool nt = false;          // specialised kernel: non-temporal HBM loads and stores
     uint64_t shared_mask = 0;
     ShmHeader hdr;
     std::vector<ShmStage> stages;
@@ -166,10 +167,11 @@ hq_status transpose_prepare(hq_state s, const int* perm, void** dev_args);
 hq_status transpose_launch(hq_state s, const void* dev_args, void* in, void* out);
 hq_status launch_dense(hq_state s, const void* d_mat, int mat_qubits, void* in, void* out);
 int sharemem_default_rb(int precision, int tile);
-hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates,
-                                 uint64_t shared_mask, int rb, hq_program* out);
+// io: bit 0 adds a leading, bit 1 a trailing stage without gates where the
+// first / last stage holds one of the three lowest tile bits in registers
+// (the HBM access then covers whole 128-byte lines per wave instruction).
 hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gates, uint64_t shared_mask,
-                        int rb, ShmPlan* out);
+                        int rb, ShmPlan* out, bool init = false, int io = 0);
 // Specialised kernel for a planned group: generate, compile (cached), load.
 hq_status jit_sharemem(const ShmPlan& plan, hq_program p);
 hq_status launch_sharemem_jit(hq_state s, hq_program p);

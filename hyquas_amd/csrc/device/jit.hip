@@ -84,12 +84,9 @@ struct Emitter {
         K = p.k;
         f32 = p.precision != HQ_C128;
         split = ((size_t)1 << K) * (f32 ? 8 : 16) > 64 * 1024;
-        // non-temporal accesses measured slower for these kernels on MI355X
-        // (7.95 vs 7.17 ms per 30-qubit pass); HQ_JIT_NT=1 turns them on
-        const char* env = getenv("HQ_JIT_NT");
-        const int ntm = env ? atoi(env) : 0;   // 1 both, 2 loads, 3 stores
-        nt_ld = ntm == 1 || ntm == 2;
-        nt_st = ntm == 1 || ntm == 3;
+        // non-temporal HBM accesses: chosen per group (they pay only where a
+        // wave instruction covers whole 128-byte lines)
+        nt_ld = nt_st = p.nt;
         const char* bm = getenv("HQ_JIT_BMAP");
         bmap = bm ? atoi(bm) : 0;
         if ((bmap == 2 && P.hdr.num_block_bits < 3) || (bmap == 1 && P.hdr.num_block_bits == 0)) bmap = 0;
@@ -642,14 +639,17 @@ hq_status launch_sharemem_jit(hq_state s, hq_program p)
 }  // namespace hq
 
 extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gates, int num_gates,
-                                        uint64_t shared_mask, int reg_qubits, char* buf, size_t cap, size_t* len,
-                                        double* coef, size_t coef_cap, size_t* num_coef, int* shape)
+                                        uint64_t shared_mask, int reg_qubits, int variant, char* buf, size_t cap,
+                                        size_t* len, double* coef, size_t coef_cap, size_t* num_coef, int* shape)
 {
     HQ_CHECK(n >= 1 && n <= 36, HQ_ERR_ARG, "hq_sharemem_source: num_qubits must be in [1, 36]");
     HQ_CHECK(precision == HQ_C128 || precision == HQ_C64, HQ_ERR_ARG, "hq_sharemem_source: bad precision");
+    HQ_CHECK(variant == 0 || variant == 1, HQ_ERR_ARG, "hq_sharemem_source: variant must be 0 or 1");
     hq::ShmPlan plan;
-    hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan);
+    hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan, false,
+                                     variant == 1 ? 3 : 0);
     if (st != HQ_OK) return st;
+    plan.nt = variant == 1;
     std::vector<double> cf;
     std::string src = hq::jit_source(plan, &cf);
     if (src.empty()) return HQ_ERR_UNSUPPORTED;

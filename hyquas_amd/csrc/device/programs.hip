@@ -207,27 +207,93 @@ hq_status hq_apply_dense(hq_state s, const double* mat, int mat_qubits)
 
 namespace hq {
 
-hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
-                                  std::vector<hq_program>* out)
+// each variant of every group is launched `rounds` times, interleaved with
+// the other variant; its fastest launch counts
+static hq_status time_variants(hq_state s, const std::vector<hq_program>& a, const std::vector<hq_program>& b,
+                               int rounds, std::vector<float>* ta, std::vector<float>* tb)
 {
-    out->assign(reqs.size(), nullptr);
-    for (size_t i = 0; i < reqs.size(); i++) {
-        hq_status st = build_sharemem_program(s, reqs[i].gates.data(), (int)reqs[i].gates.size(), reqs[i].mask,
-                                              reg_qubits, &(*out)[i]);
-        if (st != HQ_OK) {
-            for (auto p : *out) hq_program_destroy(p);
-            out->assign(reqs.size(), nullptr);
-            return st;
-        }
-        (*out)[i]->plan->init = reqs[i].init;
+    const size_t n = a.size();
+    ta->assign(n, 1e30f);
+    tb->assign(n, 1e30f);
+    hipEvent_t e0, e1;
+    HQ_HIP(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error(std::string("tune: ") + hipGetErrorString(he));
+        return HQ_ERR_HIP;
     }
-    if (!jit || reqs.empty()) return HQ_OK;
-    std::vector<const ShmPlan*> plans;
-    for (auto p : *out) plans.push_back(p->plan);
-    unsigned hw = std::thread::hardware_concurrency();
-    const char* e = getenv("OMP_NUM_THREADS");
-    int threads = e && atoi(e) > 0 ? atoi(e) : (int)(hw ? hw : 4);
-    return jit_sharemem_many(plans, *out, std::min(threads, 16));
+    const bool prof = s->profiling;
+    s->profiling = false;
+    hq_status st = HQ_OK;
+    for (int r = 0; r < rounds && st == HQ_OK; r++) {
+        for (size_t i = 0; i < n && st == HQ_OK; i++) {
+            for (int v = 0; v < 2 && st == HQ_OK; v++) {
+                hq_program p = v == 0 ? a[i] : b[i];
+                if ((he = hipEventRecord(e0, s->stream)) != hipSuccess) break;
+                st = hq_program_launch(s, p);
+                if (st != HQ_OK) break;
+                if ((he = hipEventRecord(e1, s->stream)) != hipSuccess) break;
+                if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
+                float ms = 0;
+                if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+                float& t = v == 0 ? (*ta)[i] : (*tb)[i];
+                t = std::min(t, ms);
+            }
+        }
+        if (he != hipSuccess) {
+            set_error(std::string("tune: ") + hipGetErrorString(he));
+            st = HQ_ERR_HIP;
+        }
+    }
+    s->profiling = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
+                                  std::vector<hq_program>* out, bool tune)
+{
+    tune = tune && jit && !reqs.empty();
+    const int variants = tune ? 2 : 1;
+    std::vector<std::vector<hq_program>> built(variants, std::vector<hq_program>(reqs.size(), nullptr));
+    auto destroy_all = [&]() {
+        for (auto& v : built)
+            for (auto p : v) hq_program_destroy(p);
+    };
+    out->assign(reqs.size(), nullptr);
+    for (int v = 0; v < variants; v++) {
+        for (size_t i = 0; i < reqs.size(); i++) {
+            hq_status st = build_sharemem_program(s, reqs[i].gates.data(), (int)reqs[i].gates.size(), reqs[i].mask,
+                                                  reg_qubits, &built[v][i], reqs[i].init, v);
+            if (st != HQ_OK) { destroy_all(); return st; }
+        }
+    }
+    if (jit) {
+        std::vector<const ShmPlan*> plans;
+        std::vector<hq_program> progs;
+        for (auto& v : built)
+            for (auto p : v) { plans.push_back(p->plan); progs.push_back(p); }
+        unsigned hw = std::thread::hardware_concurrency();
+        const char* e = getenv("OMP_NUM_THREADS");
+        int threads = e && atoi(e) > 0 ? atoi(e) : (int)(hw ? hw : 4);
+        hq_status st = jit_sharemem_many(plans, progs, std::min(threads, 16));
+        if (st != HQ_OK) { destroy_all(); return st; }
+    }
+    if (tune) {
+        std::vector<float> t0, t1;
+        hq_status st = time_variants(s, built[0], built[1], 2, &t0, &t1);
+        if (st != HQ_OK) { destroy_all(); return st; }
+        for (size_t i = 0; i < reqs.size(); i++) {
+            const int keep = t1[i] < t0[i] ? 1 : 0;
+            (*out)[i] = built[keep][i];
+            hq_program_destroy(built[1 - keep][i]);
+        }
+        return HQ_OK;
+    }
+    *out = built[0];
+    return HQ_OK;
 }
 
 }  // namespace hq

@@ -166,7 +166,7 @@ int rank3(const std::vector<uint32_t>& v)
 }  // namespace
 
 hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
-                        ShmPlan* plan)
+                        ShmPlan* plan, bool init, int io)
 {
     HQ_CHECK(num_gates >= 0 && (num_gates == 0 || gates), HQ_ERR_ARG, "sharemem: bad gate array");
     const size_t amp_bytes = precision == HQ_C128 ? 16 : 8;
@@ -225,6 +225,24 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
         std::vector<int> ops;
     };
     std::vector<Stage> stages;
+    // thread bits: lanes 0..2 get three bits with independent swizzle
+    // columns (lowest first), the rest follow in ascending order
+    auto assign_lanes = [&](Stage& S, uint32_t regset) {
+        std::vector<int> free_bits;
+        for (int b = 0; b < k; b++)
+            if (!((regset >> b) & 1)) free_bits.push_back(b);
+        std::vector<int> lane3;
+        std::vector<uint32_t> cols;
+        for (int b : free_bits) {
+            if (lane3.size() == 3) break;
+            cols.push_back(swz_column(b));
+            if (rank3(cols) == (int)cols.size()) lane3.push_back(b);
+            else cols.pop_back();
+        }
+        for (int b : lane3) S.thr_bits.push_back(b);
+        for (int b : free_bits)
+            if (std::find(lane3.begin(), lane3.end(), b) == lane3.end()) S.thr_bits.push_back(b);
+    };
     std::vector<int> pending(low.size());
     for (size_t i = 0; i < low.size(); i++) pending[i] = (int)i;
     // a tile above 64 KiB is exchanged in two LDS halves by the specialised
@@ -268,26 +286,37 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
             if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
         for (int b = 0; b < k; b++)
             if ((regset >> b) & 1) S.reg_bits.push_back(b);
-        // thread bits: lanes 0..2 get three bits with independent swizzle
-        // columns (lowest first), the rest follow in ascending order
-        std::vector<int> free_bits;
-        for (int b = 0; b < k; b++)
-            if (!((regset >> b) & 1)) free_bits.push_back(b);
-        std::vector<int> lane3;
-        std::vector<uint32_t> cols;
-        for (int b : free_bits) {
-            if (lane3.size() == 3) break;
-            cols.push_back(swz_column(b));
-            if (rank3(cols) == (int)cols.size()) lane3.push_back(b);
-            else cols.pop_back();
-        }
-        for (int b : lane3) S.thr_bits.push_back(b);
-        for (int b : free_bits)
-            if (std::find(lane3.begin(), lane3.end(), b) == lane3.end()) S.thr_bits.push_back(b);
+        assign_lanes(S, regset);
         prev_reg = regset;
         stages.push_back(std::move(S));
         pending.swap(rest);
     } while (!pending.empty());
+
+    // HBM-facing stages: the first stage's lanes load the tile and the last
+    // stage's lanes store it.  When one of them holds one of the three lowest
+    // tile bits in registers, a wave's access covers runs shorter than 128 B;
+    // an added stage without gates (register bits = highest tile bits, plus
+    // the neighbour's highest register bit for split exchanges) moves the
+    // HBM access to full-line lanes at the cost of one LDS exchange.
+    auto short_runs = [&](const Stage& st) {
+        for (int b : st.reg_bits)
+            if (b < 3) return true;
+        return false;
+    };
+    auto make_io = [&](const Stage& nb) {
+        Stage S;
+        uint32_t regset = 0;
+        int cnt = 0;
+        if (split_lds) { regset |= 1u << nb.reg_bits.back(); cnt++; }
+        for (int b = k - 1; b >= 0 && cnt < rb; b--)
+            if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
+        for (int b = 0; b < k; b++)
+            if ((regset >> b) & 1) S.reg_bits.push_back(b);
+        assign_lanes(S, regset);
+        return S;
+    };
+    if ((io & 1) && !init && short_runs(stages.front())) stages.insert(stages.begin(), make_io(stages.front()));
+    if ((io & 2) && short_runs(stages.back())) stages.push_back(make_io(stages.back()));
 
     // serialize
     std::vector<ShmOp> ops;
@@ -366,13 +395,16 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
 }
 
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
-                                 hq_program* out)
+                                 hq_program* out, bool init, int variant)
 {
     HQ_CHECK(s && out, HQ_ERR_ARG, "sharemem: null argument");
     *out = nullptr;
     ShmPlan plan;
-    hq_status st = plan_sharemem(s->n, s->precision, gates, num_gates, shared_mask, rb_req, &plan);
+    hq_status st = plan_sharemem(s->n, s->precision, gates, num_gates, shared_mask, rb_req, &plan, init,
+                                 variant == 1 ? 3 : 0);
     if (st != HQ_OK) return st;
+    plan.init = init;
+    plan.nt = variant == 1;
     const auto& dst = plan.stages;
     const auto& ops = plan.ops;
     size_t bytes = sizeof(ShmHeader) + dst.size() * sizeof(ShmStage) + ops.size() * sizeof(ShmOp);

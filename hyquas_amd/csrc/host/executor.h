@@ -11,6 +11,11 @@ struct ExecParams {
     int regQubits = 0;    // register qubits per thread (0: default for the tile)
     int jit = -1;         // specialised kernels: 0 off, 1 on, -1 from jitMinQubits
     int jitMinQubits = 22;
+    // per-group choice between the plain specialised kernel and the whole-line
+    // non-temporal variant, by timing both on the state: 0 off, 1 on, -1 with
+    // specialised kernels from tuneMinQubits
+    int tune = -1;
+    int tuneMinQubits = 26;
 };
 
 class Executor {

@@ -10,17 +10,22 @@
 namespace hq {
 void set_error(const std::string& msg);
 // hq_program_create_sharemem with an explicit register/thread split
-// (reg_qubits = 0: default for the tile size).
+// (reg_qubits = 0: default for the tile size); init: the group builds
+// |0...0> in registers instead of loading the state; variant 1: whole-line
+// HBM stages (added stages without gates where needed) with non-temporal
+// loads and stores in the specialised kernel.
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask,
-                                 int reg_qubits, hq_program* out);
+                                 int reg_qubits, hq_program* out, bool init = false, int variant = 0);
 
 // Many OShareMem programs at once; with jit, every group also gets a
-// structure-specialised kernel, compiled in parallel.
+// structure-specialised kernel, compiled in parallel.  With tune (jit only),
+// both program variants of every group are compiled and timed on the state
+// (its contents are overwritten) and the faster one is kept.
 struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;
     bool init = false;   // first group of a run: build |0...0> instead of loading the state
 };
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
-                                  std::vector<hq_program>* out);
+                                  std::vector<hq_program>* out, bool tune = false);
 }  // namespace hq

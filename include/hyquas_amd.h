@@ -172,10 +172,14 @@ hq_status hq_program_is_specialized(hq_program p, int *yes);
  * source of its specialised kernel (extern "C" hq_group(C *psi, const R *cf),
  * one block of 2^thread_qubits threads per tile, 2^(n - tile) blocks).
  * coef receives the runtime coefficient array (cf), shape[0..2] the tile,
- * register and thread qubit counts.  For inspection, caching and CPU tests. */
+ * register and thread qubit counts.  variant 0 is the plain kernel, 1 the
+ * whole-line non-temporal one (stages without gates added where the first
+ * or last stage would access HBM in pieces shorter than 128 bytes), the two
+ * kernels the "tune" option chooses between.  For inspection, caching and
+ * CPU tests. */
 hq_status hq_sharemem_source(int num_qubits, int precision, const hq_gate *gates, int num_gates,
-                             uint64_t shared_mask, int reg_qubits, char *buf, size_t cap, size_t *len,
-                             double *coef, size_t coef_cap, size_t *num_coef, int *shape);
+                             uint64_t shared_mask, int reg_qubits, int variant, char *buf, size_t cap,
+                             size_t *len, double *coef, size_t coef_cap, size_t *num_coef, int *shape);
 
 /* Describe a compiled OShareMem program: tile qubits, register stages and
  * lowered op count (for tests and the cost model). */
@@ -238,6 +242,10 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *   "window"      gates considered per group (default 512)
  *   "jit"         specialised OShareMem kernels: 0 off, 1 on, -1 auto
  *                 (default: on from 22 qubits)
+ *   "tune"        per-group choice, timed on the state at compile, between
+ *                 the plain specialised kernel and a whole-line non-temporal
+ *                 one: 0 off, 1 on, -1 auto (default: on with specialised
+ *                 kernels from 26 qubits); compile overwrites the state
  *   "profile_events" 1 = time every launch with HIP events
  * Must be set before compile. */
 hq_status hq_circuit_set_option(hq_circuit c, const char *key, int64_t value);

@@ -16,6 +16,9 @@ std::barrier<>* g_bar;
 #define __launch_bounds__(n)
 #define __shared__ static
 #define __restrict__
+#define __device__
+#define __forceinline__ inline
+#define amdgpu_waves_per_eu(n)
 #include GROUP_SRC
 int main(int argc, char** argv) {
   int n = atoi(argv[1]), k = atoi(argv[2]), nt = atoi(argv[3]);

@@ -78,6 +78,27 @@ def test_random_layers_jit_and_interpreter(gpu, jit):
     assert rel_err(c.logical_state(), want) <= 1e-10
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tuned_kernels_equal_plain(gpu, prec):
+    # the tuned schedule mixes plain and whole-line non-temporal kernels; both
+    # do the same arithmetic in the same order, so the states agree bit for bit
+    n = 22
+    states = []
+    for tune in (0, 1):
+        c = gpu.Circuit(n).add_random_layers(12, 9)
+        c.set_option("jit", 1)
+        c.set_option("precision", prec)
+        c.set_option("tune", tune)
+        c.compile()
+        c.run(copy_back=False, destroy=False)
+        states.append(c.logical_state(prec))
+        c.close()
+    assert np.array_equal(states[0], states[1])
+    if prec == 0:
+        want = oracle.run_plain(to_oracle(random_layers(n, 12, 9)), n)
+        assert rel_err(states[1], want) <= 1e-10
+
+
 def test_random_layers_full_vector(gpu):
     n = 18
     c = gpu.Circuit(n).add_random_layers(12, 3)

@@ -15,23 +15,34 @@ from helpers import group_for_mask as _group_for_mask, rand_state as _rand_state
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-@pytest.mark.parametrize("n,k,seed,prec,rb", [(10, 9, 31, 0, 0), (11, 10, 32, 0, 3), (12, 11, 33, 1, 0),
-                                              (11, 11, 34, 0, 5), (14, 13, 35, 0, 4), (15, 13, 36, 0, 5),
-                                              (14, 13, 37, 1, 5)])
-def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb):
+@pytest.mark.parametrize("n,k,seed,prec,rb,variant", [
+    (10, 9, 31, 0, 0, 0), (11, 10, 32, 0, 3, 0), (12, 11, 33, 1, 0, 0), (11, 11, 34, 0, 5, 0),
+    (14, 13, 35, 0, 4, 0), (15, 13, 36, 0, 5, 0), (14, 13, 37, 1, 5, 0),
+    (11, 10, 32, 0, 3, 1), (14, 13, 35, 0, 4, 1), (15, 13, 36, 0, 5, 1), (14, 13, 37, 1, 5, 1)])
+def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, variant):
     rs = np.random.default_rng(seed)
     rest = rs.choice(np.arange(3, n), k - 3, replace=False)
     mask = 0
     for q in [0, 1, 2] + [int(x) for x in rest]:
         mask |= 1 << q
     g = _group_for_mask(n, mask, 90, seed)
-    src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, g, mask, prec, rb)
+    src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, g, mask, prec, rb, variant)
     assert tile == k and (rb == 0 or rbits == rb)
+    # variant 1: non-temporal accesses (clang vector builtins: host clang runs them)
+    cxx = "g++"
+    if variant == 1:
+        assert "__builtin_nontemporal_load" in src and "__builtin_nontemporal_store" in src
+        if not os.path.exists(CLANG):
+            pytest.skip("needs ROCm clang++ for the non-temporal builtins")
+        cxx = CLANG
     ksrc = tmp_path / "kernel.hip"
     ksrc.write_text(src)
     exe = tmp_path / "harness"
-    subprocess.run(["g++", "-O1", "-std=c++20", "-pthread", f'-DGROUP_SRC="{ksrc}"',
+    subprocess.run([cxx, "-O1", "-std=c++20", "-pthread", "-Wno-unknown-attributes", f'-DGROUP_SRC="{ksrc}"',
                     os.path.join(HERE, "jit_harness.cpp"), "-o", str(exe)], check=True)
     psi0 = _rand_state(n, seed)
     dt = np.complex128 if prec == 0 else np.complex64
