@@ -86,6 +86,10 @@ def main():
     ap.add_argument("--share-qubits", type=int, default=0, help="OShareMem tile qubits (0: default)")
     ap.add_argument("--reg-qubits", type=int, default=0, help="register qubits per thread (0: default)")
     ap.add_argument("--coalesce", type=int, default=0, help="low qubits always in the tile (0: default)")
+    ap.add_argument("--beam", type=int, default=0, help="partitions kept by the beam search (0: default)")
+    ap.add_argument("--starts", type=int, default=0, help="merge starts per beam step (0: default)")
+    ap.add_argument("--tune", type=int, choices=[-1, 0, 1], default=-1,
+                    help="per-group kernel variant timing at compile: -1 auto, 0 off, 1 on")
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
                     help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
@@ -134,6 +138,11 @@ def main():
     if a.coalesce:
         c.set_option("coalesce", a.coalesce)
     c.set_option("jit", a.jit)
+    c.set_option("tune", a.tune)
+    if a.beam:
+        c.set_option("beam", a.beam)
+    if a.starts:
+        c.set_option("starts", a.starts)
     jit_on = a.jit == 1 or (a.jit == -1 and n >= 22)
     if a.cost_model:
         c.load_cost_model(a.cost_model)

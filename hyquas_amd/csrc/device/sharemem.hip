@@ -293,14 +293,16 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
     } while (!pending.empty());
 
     // HBM-facing stages: the first stage's lanes load the tile and the last
-    // stage's lanes store it.  When one of them holds one of the three lowest
-    // tile bits in registers, a wave's access covers runs shorter than 128 B;
-    // an added stage without gates (register bits = highest tile bits, plus
-    // the neighbour's highest register bit for split exchanges) moves the
-    // HBM access to full-line lanes at the cost of one LDS exchange.
+    // stage's lanes store it.  When one of them holds one of the lowest tile
+    // bits of a 128-byte line (3 for complex128, 4 for complex64) in
+    // registers, a wave's access covers each line in pieces; an added stage
+    // without gates (register bits = highest tile bits, plus the neighbour's
+    // highest register bit for split exchanges) moves the HBM access to
+    // whole-line lanes at the cost of one LDS exchange.
+    const int line_bits = amp_bytes == 16 ? 3 : 4;
     auto short_runs = [&](const Stage& st) {
         for (int b : st.reg_bits)
-            if (b < 3) return true;
+            if (b < line_bits) return true;
         return false;
     };
     auto make_io = [&](const Stage& nb) {
