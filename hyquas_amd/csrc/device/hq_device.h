@@ -107,9 +107,10 @@ constexpr uint32_t SHM_FLAG_CONST = 1;
 
 constexpr int SHM_MAX_TILE = 14;
 constexpr int SHM_MAX_TB = 9;
+constexpr int SHM_MAX_RB = 6;   // 6 only for complex64 (64 amplitudes = 128 VGPRs per thread)
 
 struct ShmStage {
-    uint32_t reg_bits[5];      // tile bit of register-index bit i (RB used)
+    uint32_t reg_bits[SHM_MAX_RB];  // tile bit of register-index bit i (RB used)
     uint32_t thr_bits[SHM_MAX_TB];  // tile bit of thread-index bit i (TB used)
     uint32_t op_begin, op_end;
 };

@@ -61,18 +61,19 @@ static hq_status launch_by_shape(hq_state s, hq_program p)
     HQ_SHM_CASE(1, 6) HQ_SHM_CASE(2, 6) HQ_SHM_CASE(3, 6) HQ_SHM_CASE(4, 6) HQ_SHM_CASE(5, 6)
     HQ_SHM_CASE(2, 7) HQ_SHM_CASE(3, 7) HQ_SHM_CASE(4, 7) HQ_SHM_CASE(5, 7)
     HQ_SHM_CASE(3, 8) HQ_SHM_CASE(4, 8) HQ_SHM_CASE(5, 8) HQ_SHM_CASE(3, 9) HQ_SHM_CASE(4, 9)
+    if constexpr (sizeof(R) == 4) { HQ_SHM_CASE(6, 8) }   // 14-qubit complex64 tiles
 #undef HQ_SHM_CASE
     set_error("sharemem: no kernel for this register/thread split");
     return HQ_ERR_UNSUPPORTED;
 }
 
-static bool shape_supported(int rb, int tb)
+static bool shape_supported(int rb, int tb, int precision)
 {
     static const int shapes[][2] = {{1, 6}, {2, 6}, {3, 6}, {4, 6}, {5, 6}, {2, 7}, {3, 7},
                                     {4, 7}, {5, 7}, {3, 8}, {4, 8}, {5, 8}, {3, 9}, {4, 9}};
     for (auto& sh : shapes)
         if (sh[0] == rb && sh[1] == tb) return true;
-    return false;
+    return precision == HQ_C64 && rb == 6 && tb == 8;
 }
 
 hq_status launch_sharemem(hq_state s, hq_program p)
@@ -86,8 +87,10 @@ int sharemem_default_rb(int precision, int tile)
     // amplitudes per thread at the default 12/13-qubit tiles
     if (tile <= 10) return std::max(1, tile - 6);
     if (precision == HQ_C128) return tile >= 13 ? 5 : 4;
-    // complex64: 256 threads x 16 amplitudes at 12 qubits measured fastest
-    return tile >= 13 ? 5 : tile == 12 ? 4 : tile - 6;
+    // complex64: 256 threads x 16 amplitudes at 12 qubits measured fastest;
+    // 14-qubit tiles hold 64 amplitudes per thread (the register bytes of a
+    // 13-qubit complex128 tile)
+    return tile >= 14 ? 6 : tile == 13 ? 5 : tile == 12 ? 4 : tile - 6;
 }
 
 // ---- host side: lowering + register-stage planning -------------------------------------------
@@ -177,7 +180,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
     HQ_CHECK(((size_t)1 << k) * amp_bytes <= 160 * 1024, HQ_ERR_ARG, "sharemem: tile exceeds the 160 KiB LDS");
     const int rb = rb_req > 0 ? rb_req : sharemem_default_rb(precision, k);
     const int tb = k - rb;
-    HQ_CHECK(shape_supported(rb, tb), HQ_ERR_UNSUPPORTED, "sharemem: unsupported register/thread split");
+    HQ_CHECK(shape_supported(rb, tb, precision), HQ_ERR_UNSUPPORTED, "sharemem: unsupported register/thread split");
 
     int tile_of[64], block_of[64];
     std::vector<int> tile_pos, block_pos;

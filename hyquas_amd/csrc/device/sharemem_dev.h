@@ -72,8 +72,10 @@ __device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const ShmOp* __r
         case 1: if constexpr (RB > 1) reg_op<R, RB, 1>(a, cls, creg, m); break;
         case 2: if constexpr (RB > 2) reg_op<R, RB, 2>(a, cls, creg, m); break;
         case 3: if constexpr (RB > 3) reg_op<R, RB, 3>(a, cls, creg, m); break;
-        default: if constexpr (RB > 4) reg_op<R, RB, 4>(a, cls, creg, m); break;
+        case 4: if constexpr (RB > 4) reg_op<R, RB, 4>(a, cls, creg, m); break;
+        default: if constexpr (RB > 5) reg_op<R, RB, 5>(a, cls, creg, m); break;
         }
+        static_assert(RB <= 6, "register positions above 5 are not dispatched");
         return;
     }
     // diagonal gate whose target is a thread bit or a block bit: one factor

@@ -130,11 +130,13 @@ void Circuit::compile()
 
 int Circuit::autoLocalSize() const
 {
-    // specialised kernels exchange a 13-qubit tile in two LDS halves and keep
-    // two blocks per CU (fewer passes, measured faster); the interpreter keeps
-    // 12-qubit tiles (64 KiB LDS)
+    // specialised kernels exchange a 128 KiB tile (13 qubits of complex128,
+    // 14 of complex64) in two LDS halves and keep two blocks per CU (fewer
+    // passes, measured faster); the interpreter keeps 12-qubit tiles (64 KiB
+    // LDS for complex128)
     const bool jit = exec.jit == 1 || (exec.jit < 0 && numQubits >= exec.jitMinQubits);
-    return jit ? 13 : 12;
+    if (!jit) return 12;
+    return params.precision == HQ_C64 ? 14 : 13;
 }
 
 static const char kSchedMagic[8] = {'H', 'Q', 'S', 'C', 'H', 'E', 'D', '1'};

@@ -233,7 +233,8 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *   "precision"   HQ_C128 / HQ_C64 (default HQ_C128)
  *   "profile"     0 = gfx950 parameters (default), 1 = reference parameters
  *                 (LOCAL_QUBIT_SIZE 10, COALESCE 3, V100 cost tables)
- *   "share_qubits" OShareMem tile qubits (default 12; 6..14)
+ *   "share_qubits" OShareMem tile qubits (default with specialised kernels:
+ *                 13 for complex128, 14 for complex64; interpreter: 12; 7..14)
  *   "coalesce"    COALESCE_GLOBAL (default 4; the reference uses 3)
  *   "mat_min", "mat_max"  TransMM group qubit range in mix mode (4, 7)
  *   "blas_mat"    BLAS_MAT_LIMIT for HQ_MODE_BLAS (default 6)

@@ -142,6 +142,27 @@ def test_c64_fidelity(gpu):
     assert 1 - fid <= 1e-5
 
 
+@pytest.mark.parametrize("jit", [0, 1])
+def test_c64_14_qubit_tiles(gpu, jit):
+    # complex64 tiles of 14 qubits: 256 threads x 64 amplitudes (two-phase
+    # exchange in the specialised kernel, one 128 KiB exchange in the interpreter)
+    n = 18
+    c = gpu.Circuit(n).add_random_layers(12, 13)
+    c.set_option("precision", gpu.HQ_C64)
+    c.set_option("mode", gpu.HQ_MODE_GROUP)
+    c.set_option("share_qubits", 14)
+    c.set_option("jit", jit)
+    c.set_option("tune", jit)
+    c.compile()
+    assert all(g["backend"] != gpu.HQ_BACKEND_PERGATE or g["num_stages"] > 0 for g in c.schedule())
+    c.run(copy_back=False, destroy=False)
+    got = c.logical_state(gpu.HQ_C64)
+    want = oracle.run_plain(to_oracle(random_layers(n, 12, 13)), n)
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+    assert 1 - fid <= 1e-5
+    assert np.max(np.abs(got - want)) <= 1e-5
+
+
 @pytest.mark.parametrize("n,layers,prec", [(30, 30, "c128"), (33, 24, "c64")])
 def test_round_trip_at_full_size(gpu, n, layers, prec):
     """The headline circuits at their full sizes, followed by their inverse,

@@ -22,7 +22,8 @@ CLANG = "/opt/rocm/llvm/bin/clang++"
 @pytest.mark.parametrize("n,k,seed,prec,rb,variant", [
     (10, 9, 31, 0, 0, 0), (11, 10, 32, 0, 3, 0), (12, 11, 33, 1, 0, 0), (11, 11, 34, 0, 5, 0),
     (14, 13, 35, 0, 4, 0), (15, 13, 36, 0, 5, 0), (14, 13, 37, 1, 5, 0),
-    (11, 10, 32, 0, 3, 1), (14, 13, 35, 0, 4, 1), (15, 13, 36, 0, 5, 1), (14, 13, 37, 1, 5, 1)])
+    (11, 10, 32, 0, 3, 1), (14, 13, 35, 0, 4, 1), (15, 13, 36, 0, 5, 1), (14, 13, 37, 1, 5, 1),
+    (15, 14, 38, 1, 0, 0), (16, 14, 39, 1, 6, 1)])
 def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, variant):
     rs = np.random.default_rng(seed)
     rest = rs.choice(np.arange(3, n), k - 3, replace=False)
