@@ -233,6 +233,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _progress(msg):
+    # one line per finished step on stderr: long forms stay visibly alive
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _fidelity(hq, spec, kinds="hea"):
     """1 - |<psi64|psi128>|^2 for one circuit (OShareMem path, both precisions)."""
     from hyquas_amd.serving import build_circuit
@@ -284,6 +289,7 @@ def main_batch(a, hq):
                      "max_norm_error": float(np.max(np.abs(ex[:, 0] - 1)))}
         b.close()
         e.close()
+        _progress(f"batch: {name} {res[name]['circuits_per_s']} circuits/s")
     # plain FP64, one circuit at a time (serial per-gate kernels), evenly spaced sample
     idx = list(range(0, a.count, max(1, a.count // max(1, a.plain_sample))))[: a.plain_sample]
     t0 = time.perf_counter()
@@ -298,6 +304,7 @@ def main_batch(a, hq):
         c.close()
     plain_s = time.perf_counter() - t0
     plain_cps = len(idx) / plain_s
+    _progress(f"batch: plain FP64 {plain_cps:.1f} circuits/s")
     dzz_plain = float(np.max(np.abs(np.array(pz) - zz["c128"][idx])))
     fid_idx = sorted(set([int(np.argmax([s[0] for s in specs]))] + idx[:4]))
     fids = [_fidelity(hq, specs[i]) for i in fid_idx]
@@ -337,8 +344,10 @@ def main_serve(a, hq):
         recs, wall = serve(hq, e, reqs, circuits)
         st = e.stats()
         e.close()
+        _progress(f"serve: rate {rate} engine done in {wall:.1f} s")
         circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in reqs]
         precs, pwall = serve_plain(hq, reqs, circuits)
+        _progress(f"serve: rate {rate} plain done in {pwall:.1f} s")
         dz = [abs(x["result"]["z"] - y["result"]["z"]) for x, y in zip(recs, precs) if not x["error"]]
         topk_same = sum(1 for x, y, r in zip(recs, precs, reqs) if r["output"] == "topk" and not x["error"]
                         and x["result"]["topk"][0][:4] == list(y["result"]["topk"][0][:4]))
@@ -394,7 +403,9 @@ def main_precision(a, hq):
         t0 = time.perf_counter()
         c.compile()
         comp = time.perf_counter() - t0
+        _progress(f"precision: {n}q complex64 compiled in {comp:.1f} s")
         sec, launches = _timed_run(hq, c, steps, warm)
+        _progress(f"precision: {n}q complex64 {sec:.4f} s per simulation")
         shm = [ms for k, ms in launches if k == hq.KIND_SHAREMEM]
         per_step = len(shm) // steps
         alg = 2.0 * (1 << n) * 8 * (per_step - 0.5) / per_step
@@ -413,6 +424,7 @@ def main_precision(a, hq):
     ref.run(copy_back=False, destroy=False)
     ref_state = ref.state_view(hq.HQ_C128)
     psi_ref = ref_state.download()
+    _progress("precision: 30q complex128 reference done")
     errs = {}
     for name, mode, mat in (("c64_group", hq.HQ_MODE_GROUP, 0), ("c64_transmm_k64", hq.HQ_MODE_BLAS, 6),
                             ("c64_transmm_k128", hq.HQ_MODE_BLAS, 7)):
@@ -424,6 +436,7 @@ def main_precision(a, hq):
             c.set_option("blas_mat", mat)
         c.compile()
         sec, launches = _timed_run(hq, c, 1, 1 if mode == hq.HQ_MODE_GROUP else 0)
+        _progress(f"precision: 30q {name} {sec:.4f} s per simulation")
         # squared norm from the device's FP64 (Kahan) reduction
         if mode == hq.HQ_MODE_GROUP:
             ov = ref_state.inner(c.state_view(hq.HQ_C64))

@@ -154,6 +154,17 @@ struct Emitter {
         for (int j = 0; j < NR; j++) name[j] = j;
     }
 
+    // complex64, whole-line variant: register-index bit holding the lowest
+    // state qubit in stage st, if any; registers j and j | (1 << bit) are then
+    // neighbours in HBM and move as one 16-byte access
+    int pair_pos(const ShmStage& st) const
+    {
+        if (!f32 || !P.nt) return -1;
+        for (int i = 0; i < P.rb; i++)
+            if (P.hdr.tile_phys[st.reg_bits[i]] == 1) return i;
+        return -1;
+    }
+
     // physical offset (as a literal) of register index j in stage s
     uint64_t reg_off(const ShmStage& st, int j) const
     {
@@ -336,7 +347,10 @@ struct Emitter {
         const int nst = (int)P.stages.size();
         o << "typedef " << (f32 ? "float" : "double") << " R;\n";
         o << "struct __attribute__((aligned(" << (f32 ? 8 : 16) << "))) C { R x, y; };\n";
+        const int pp_ld = P.init ? -1 : pair_pos(P.stages[0]);
+        const int pp_st = pair_pos(P.stages[nst - 1]);
         if (nt_ld || nt_st) o << "typedef R V2 __attribute__((ext_vector_type(2)));\n";
+        if (pp_ld >= 0 || pp_st >= 0) o << "typedef float F4 __attribute__((ext_vector_type(4)));\n";
         if (nt_ld)
             o << "__device__ __forceinline__ C ld(const C* p) { const V2 v = __builtin_nontemporal_load((const V2*)p); "
                  "C c; c.x = v.x; c.y = v.y; return c; }\n";
@@ -380,6 +394,16 @@ struct Emitter {
             } else {
                 o << "    const C* __restrict__ src = psi + (base + lp);\n";
                 for (int j = 0; j < NR; j++) {
+                    if (pp_ld >= 0) {
+                        if ((j >> pp_ld) & 1) continue;
+                        const int jh = j | (1 << pp_ld);
+                        o << "    { const F4 v = ";
+                        if (nt_ld) o << "__builtin_nontemporal_load((const F4*)(src + " << reg_off(st, j) << "ull));";
+                        else o << "*(const F4*)(src + " << reg_off(st, j) << "ull);";
+                        o << " r" << j << ".x = v.x; r" << j << ".y = v.y; r" << jh << ".x = v.z; r" << jh
+                          << ".y = v.w; }\n";
+                        continue;
+                    }
                     if (nt_ld) o << "    r" << j << " = ld(src + " << reg_off(st, j) << "ull);\n";
                     else o << "    r" << j << " = src[" << reg_off(st, j) << "ull];\n";
                 }
@@ -416,6 +440,15 @@ struct Emitter {
             lane_phys(st, "lq");
             o << "    C* __restrict__ dst = psi + (base + lq);\n";
             for (int j = 0; j < NR; j++) {
+                if (pp_st >= 0) {
+                    if ((j >> pp_st) & 1) continue;
+                    const int jh = j | (1 << pp_st);
+                    o << "    { F4 v; v.x = " << r(j) << ".x; v.y = " << r(j) << ".y; v.z = " << r(jh) << ".x; v.w = "
+                      << r(jh) << ".y; ";
+                    if (nt_st) o << "__builtin_nontemporal_store(v, (F4*)(dst + " << reg_off(st, j) << "ull)); }\n";
+                    else o << "*(F4*)(dst + " << reg_off(st, j) << "ull) = v; }\n";
+                    continue;
+                }
                 if (nt_st) o << "    st(dst + " << reg_off(st, j) << "ull, " << r(j) << ");\n";
                 else o << "    dst[" << reg_off(st, j) << "ull] = " << r(j) << ";\n";
             }

@@ -301,18 +301,22 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
     // registers, a wave's access covers each line in pieces; an added stage
     // without gates (register bits = highest tile bits, plus the neighbour's
     // highest register bit for split exchanges) moves the HBM access to
-    // whole-line lanes at the cost of one LDS exchange.
-    const int line_bits = amp_bytes == 16 ? 3 : 4;
+    // whole-line lanes at the cost of one LDS exchange.  complex64 tiles that
+    // hold the lowest state qubit keep it as a register bit in such a stage:
+    // a lane then moves two neighbouring amplitudes as one 16-byte access and
+    // 8 lanes cover a 128-byte line.
+    const bool pair64 = amp_bytes == 8 && tile_pos[0] == 0;
     auto short_runs = [&](const Stage& st) {
-        for (int b : st.reg_bits)
-            if (b < line_bits) return true;
-        return false;
+        uint32_t regs = 0;
+        for (int b : st.reg_bits) regs |= 1u << b;
+        return (regs & ((1u << (amp_bytes == 16 ? 3 : 4)) - 1)) != 0;
     };
     auto make_io = [&](const Stage& nb) {
         Stage S;
         uint32_t regset = 0;
         int cnt = 0;
-        if (split_lds) { regset |= 1u << nb.reg_bits.back(); cnt++; }
+        if (pair64) { regset |= 1u; cnt++; }
+        if (split_lds && !((regset >> nb.reg_bits.back()) & 1)) { regset |= 1u << nb.reg_bits.back(); cnt++; }
         for (int b = k - 1; b >= 0 && cnt < rb; b--)
             if (!((regset >> b) & 1)) { regset |= 1u << b; cnt++; }
         for (int b = 0; b < k; b++)
