@@ -590,6 +590,9 @@ std::string jit_source(const ShmPlan& plan, std::vector<double>* coef)
 static hq_status load_program(const ShmPlan& plan, const std::vector<char>& code, const std::vector<double>& coef,
                               hq_program p)
 {
+    // a program loaded before (a retried batch) gives its module and coefficients back first
+    if (p->jit_module) { (void)hipModuleUnload(p->jit_module); p->jit_module = nullptr; p->jit_fn = nullptr; }
+    if (p->d_coef) { (void)hipFree(p->d_coef); p->d_coef = nullptr; }
     HQ_HIP(hipModuleLoadData(&p->jit_module, code.data()));
     HQ_HIP(hipModuleGetFunction(&p->jit_fn, p->jit_module, "hq_group"));
     size_t rbytes = plan.precision == HQ_C128 ? 8 : 4;

@@ -279,6 +279,15 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
         const char* e = getenv("OMP_NUM_THREADS");
         int threads = e && atoi(e) > 0 ? atoi(e) : (int)(hw ? hw : 4);
         hq_status st = jit_sharemem_many(plans, progs, std::min(threads, 16));
+        if (st != HQ_OK && tune) {
+            // the plain kernels alone (the whole-line variant is an option, not a requirement)
+            for (auto p : built[1]) hq_program_destroy(p);
+            built.resize(1);
+            tune = false;
+            plans.resize(reqs.size());
+            progs.resize(reqs.size());
+            st = jit_sharemem_many(plans, progs, std::min(threads, 16));
+        }
         if (st != HQ_OK) { destroy_all(); return st; }
     }
     if (tune) {
