@@ -81,7 +81,8 @@ def test_random_layers_jit_and_interpreter(gpu, jit):
 @pytest.mark.parametrize("prec", [0, 1])
 def test_tuned_kernels_equal_plain(gpu, prec):
     # the tuned schedule mixes plain and whole-line non-temporal kernels; both
-    # do the same arithmetic in the same order, so the states agree bit for bit
+    # apply the same operations in the same order (the compiler may still
+    # contract multiply-adds differently in the two kernels: rounding level)
     n = 22
     states = []
     for tune in (0, 1):
@@ -93,7 +94,7 @@ def test_tuned_kernels_equal_plain(gpu, prec):
         c.run(copy_back=False, destroy=False)
         states.append(c.logical_state(prec))
         c.close()
-    assert np.array_equal(states[0], states[1])
+    assert rel_err(states[1], states[0]) <= (1e-14 if prec == 0 else 1e-6)
     if prec == 0:
         want = oracle.run_plain(to_oracle(random_layers(n, 12, 9)), n)
         assert rel_err(states[1], want) <= 1e-10
