@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--tune", type=int, choices=[-1, 0, 1], default=-1,
                     help="per-group kernel variant timing at compile: -1 auto, 0 off, 1 on")
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
+    ap.add_argument("--blas-mat", type=int, default=0, help="TransMM matrix qubits in blas mode (0: default)")
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
                     help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
     ap.add_argument("--cost-model", default="", help="parameter file for the partitioner's cost model")
@@ -137,6 +138,8 @@ def main():
         c.set_option("reg_qubits", a.reg_qubits)
     if a.coalesce:
         c.set_option("coalesce", a.coalesce)
+    if a.blas_mat:
+        c.set_option("blas_mat", a.blas_mat)
     c.set_option("jit", a.jit)
     c.set_option("tune", a.tune)
     if a.beam:
@@ -198,6 +201,25 @@ def main():
     # whole-step roofline: every group needs one HBM pass (TransMM two)
     t_ideal = (groups_pg + 2 * groups_blas - (0.5 if init_fused else 0.0)) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
 
+    # TransMM groups: GEMM flops (8 K 2^n per group) over the dense kernels'
+    # time, transposes as bytes moved (read + write of the state)
+    transmm = None
+    dense = [ms for k, ms in launches if k == hq.KIND_DENSE]
+    trans = [ms for k, ms in launches if k == hq.KIND_TRANSPOSE]
+    if dense:
+        flops = sum(8.0 * (1 << g["mat_qubit"]) * (1 << n) for g in sched
+                    if g["backend"] == hq.HQ_BACKEND_BLAS) * a.steps
+        mfma_peak = 78.6 if prec == hq.HQ_C128 else 157.3
+        tflops = flops / (sum(dense) * 1e-3) / 1e12
+        transmm = {"dense_launches": len(dense), "dense_avg_ms": round(sum(dense) / len(dense), 3),
+                   "dense_TFLOPs": round(tflops, 2), "mfma_peak_TFLOPs": mfma_peak,
+                   "mfma_frac": round(tflops / mfma_peak, 4),
+                   "mat_qubits": sorted({g["mat_qubit"] for g in sched if g["backend"] == hq.HQ_BACKEND_BLAS})}
+        if trans:
+            tavg = sum(trans) / len(trans)
+            transmm.update({"transpose_launches": len(trans), "transpose_avg_ms": round(tavg, 3),
+                            "transpose_GBps": round(bytes_per_pass / (tavg * 1e-3) / 1e9, 1)})
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -224,6 +246,7 @@ def main():
                    "groups_transmm": groups_blas, "share_qubits": a.share_qubits or None,
                    "reg_qubits": a.reg_qubits or None, "mode": a.mode, "parallelism": "single GPU"},
         "roofline": roof,
+        "transmm": transmm,
         "step_hbm_roofline_frac": round(t_ideal / sec_per_step, 4),
         "device_ms_per_step": {str(k): round(v / a.steps, 3) for k, v in sorted(kinds.items())},
         "compile_s": round(compile_s, 3),
