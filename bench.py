@@ -88,8 +88,9 @@ def main():
     ap.add_argument("--coalesce", type=int, default=0, help="low qubits always in the tile (0: default)")
     ap.add_argument("--beam", type=int, default=0, help="partitions kept by the beam search (0: default)")
     ap.add_argument("--starts", type=int, default=0, help="merge starts per beam step (0: default)")
-    ap.add_argument("--tune", type=int, choices=[-1, 0, 1], default=-1,
-                    help="per-group kernel variant timing at compile: -1 auto, 0 off, 1 on")
+    ap.add_argument("--tune", type=int, choices=[-1, 0, 1, 2], default=-1,
+                    help="per-group kernel variant timing at compile: -1 auto, 0 off, 1 on, "
+                         "2 whole-line variant everywhere")
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
     ap.add_argument("--blas-mat", type=int, default=0, help="TransMM matrix qubits in blas mode (0: default)")
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,

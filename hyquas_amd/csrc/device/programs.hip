@@ -253,9 +253,11 @@ static hq_status time_variants(hq_state s, const std::vector<hq_program>& a, con
 }
 
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
-                                  std::vector<hq_program>* out, bool tune)
+                                  std::vector<hq_program>* out, int tune_mode)
 {
-    tune = tune && jit && !reqs.empty();
+    // tune_mode 1: both variants, timed; 2: the whole-line variant only
+    bool tune = tune_mode == 1 && jit && !reqs.empty();
+    const int only = tune_mode == 2 && jit ? 1 : 0;
     const int variants = tune ? 2 : 1;
     std::vector<std::vector<hq_program>> built(variants, std::vector<hq_program>(reqs.size(), nullptr));
     auto destroy_all = [&]() {
@@ -266,7 +268,7 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
     for (int v = 0; v < variants; v++) {
         for (size_t i = 0; i < reqs.size(); i++) {
             hq_status st = build_sharemem_program(s, reqs[i].gates.data(), (int)reqs[i].gates.size(), reqs[i].mask,
-                                                  reg_qubits, &built[v][i], reqs[i].init, v);
+                                                  reg_qubits, &built[v][i], reqs[i].init, tune ? v : only);
             if (st != HQ_OK) { destroy_all(); return st; }
         }
     }

@@ -179,7 +179,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
         HQ_CARG(v >= -1 && v <= 1, "jit must be -1 (auto), 0 or 1");
         C.exec.jit = (int)v;
     } else if (k == "tune") {
-        HQ_CARG(v >= -1 && v <= 1, "tune must be -1 (auto), 0 or 1");
+        HQ_CARG(v >= -1 && v <= 2, "tune must be -1 (auto), 0, 1 or 2");
         C.exec.tune = (int)v;
     } else if (k == "profile_events") {
         C.profileEvents = v != 0;

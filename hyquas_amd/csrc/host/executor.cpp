@@ -79,7 +79,7 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
     }
     const bool jit = p.jit == 1 || (p.jit < 0 && n >= p.jitMinQubits);
     std::vector<hq_program> progs;
-    const bool tune = jit && (p.tune == 1 || (p.tune < 0 && n >= p.tuneMinQubits));
+    const int tune = !jit ? 0 : p.tune >= 0 ? p.tune : (n >= p.tuneMinQubits ? 1 : 0);
     hq_status rc = hq::build_sharemem_programs(s, reqs, p.regQubits, jit, &progs, tune);
     if (rc != HQ_OK) return rc;
     for (size_t i = 0; i < owners.size(); i++) {

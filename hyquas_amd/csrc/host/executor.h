@@ -12,8 +12,9 @@ struct ExecParams {
     int jit = -1;         // specialised kernels: 0 off, 1 on, -1 from jitMinQubits
     int jitMinQubits = 22;
     // per-group choice between the plain specialised kernel and the whole-line
-    // non-temporal variant, by timing both on the state: 0 off, 1 on, -1 with
-    // specialised kernels from tuneMinQubits
+    // non-temporal variant, by timing both on the state: 0 off (plain), 1 on,
+    // 2 whole-line variant everywhere (no timing), -1: 1 with specialised
+    // kernels from tuneMinQubits, else 0
     int tune = -1;
     int tuneMinQubits = 26;
 };

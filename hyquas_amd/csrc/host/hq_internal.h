@@ -18,14 +18,15 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
                                  int reg_qubits, hq_program* out, bool init = false, int variant = 0);
 
 // Many OShareMem programs at once; with jit, every group also gets a
-// structure-specialised kernel, compiled in parallel.  With tune (jit only),
-// both program variants of every group are compiled and timed on the state
-// (its contents are overwritten) and the faster one is kept.
+// structure-specialised kernel, compiled in parallel.  tune_mode (jit only):
+// 1 compiles both program variants of every group, times them on the state
+// (its contents are overwritten) and keeps the faster; 2 uses the whole-line
+// variant for every group; 0 the plain one.
 struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;
     bool init = false;   // first group of a run: build |0...0> instead of loading the state
 };
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
-                                  std::vector<hq_program>* out, bool tune = false);
+                                  std::vector<hq_program>* out, int tune_mode = 0);
 }  // namespace hq

@@ -245,8 +245,9 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *                 (default: on from 22 qubits)
  *   "tune"        per-group choice, timed on the state at compile, between
  *                 the plain specialised kernel and a whole-line non-temporal
- *                 one: 0 off, 1 on, -1 auto (default: on with specialised
- *                 kernels from 26 qubits); compile overwrites the state
+ *                 one: 0 off, 1 on, 2 the whole-line kernel everywhere
+ *                 (untimed), -1 auto (default: on with specialised kernels
+ *                 from 26 qubits); timing overwrites the state
  *   "profile_events" 1 = time every launch with HIP events
  * Must be set before compile. */
 hq_status hq_circuit_set_option(hq_circuit c, const char *key, int64_t value);

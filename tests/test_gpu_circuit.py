@@ -79,13 +79,15 @@ def test_random_layers_jit_and_interpreter(gpu, jit):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_tuned_kernels_equal_plain(gpu, prec):
-    # the tuned schedule mixes plain and whole-line non-temporal kernels; both
-    # apply the same operations in the same order (the compiler may still
-    # contract multiply-adds differently in the two kernels: rounding level)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_tuned_kernels_equal_plain(gpu, prec, mode):
+    # mode 1: the tuned schedule mixes plain and whole-line non-temporal
+    # kernels; mode 2: whole-line kernels everywhere.  Both kernels apply the
+    # same operations in the same order (the compiler may still contract
+    # multiply-adds differently in the two: rounding level)
     n = 22
     states = []
-    for tune in (0, 1):
+    for tune in (0, mode):
         c = gpu.Circuit(n).add_random_layers(12, 9)
         c.set_option("jit", 1)
         c.set_option("precision", prec)
