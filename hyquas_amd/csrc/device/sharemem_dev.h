@@ -20,6 +20,17 @@ constexpr uint32_t TILE_READ = 4;     // write {sum p, sum p*Z, sum p*ZZ} of the
 template <typename R, int RB, int CLS, int P>
 __device__ __forceinline__ void reg_pairs(cplx<R> (&a)[1 << RB], uint32_t creg, const R* m)
 {
+    // without register controls every pair is updated: no per-pair branch
+    // (a branch per pair makes the compiler copy both amplitudes for the
+    // path that skips the update)
+    if (creg == 0) {
+#pragma unroll
+        for (int j = 0; j < (1 << RB); j++) {
+            if (j & (1 << P)) continue;
+            pair_update<CLS, R>(a[j], a[j | (1 << P)], m);
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < (1 << RB); j++) {
         if (j & (1 << P)) continue;
@@ -173,14 +184,19 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
 
     for (uint32_t s = 0; s < nst; s++) {
         if (s > 0) {
-            // exchange: write under the old mapping, read under the new one
+            // exchange: write under the old mapping, read under the new one.
+            // The swizzle is linear over xor, so a register's LDS index is the
+            // lane's swizzled index xor the swizzled bits of the register index
             const uint32_t st = swz<K>(et, col);
+            uint32_t srb[RB];
+#pragma unroll
+            for (int i = 0; i < RB; i++) srb[i] = swz<K>(rbit[i], col);
 #pragma unroll
             for (int j = 0; j < NR; j++) {
-                uint32_t off = 0;
+                uint32_t so = 0;
 #pragma unroll
-                for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rbit[i];
-                lds[st ^ swz<K>(off, col)] = a[j];
+                for (int i = 0; i < RB; i++) if ((j >> i) & 1) so ^= srb[i];
+                lds[st ^ so] = a[j];
             }
             __syncthreads();
 #pragma unroll
@@ -191,11 +207,13 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
                 if ((tid >> i) & 1) et |= 1u << ST[s].thr_bits[i];
             const uint32_t st2 = swz<K>(et, col);
 #pragma unroll
-            for (int j = 0; j < NR; j++) {
-                uint32_t off = 0;
+            for (int i = 0; i < RB; i++) srb[i] = swz<K>(rbit[i], col);
 #pragma unroll
-                for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rbit[i];
-                a[j] = lds[st2 ^ swz<K>(off, col)];
+            for (int j = 0; j < NR; j++) {
+                uint32_t so = 0;
+#pragma unroll
+                for (int i = 0; i < RB; i++) if ((j >> i) & 1) so ^= srb[i];
+                a[j] = lds[st2 ^ so];
             }
             // the next exchange overwrites lds: every lane must have read first
             if (s + 1 < nst) __syncthreads();
