@@ -378,6 +378,27 @@ struct Emitter {
             o << "    base |= ((b >> " << j << ") & " << hex64((1ull << len) - 1) << ") << " << p << ";\n";
             j += len - 1;
         }
+        if (P.init && P.hdr.num_block_bits > 0) {
+            // first group of a run: every tile but the one holding |0...0>
+            // starts and (the group's gates being linear) stays zero
+            const ShmStage& st = P.stages[nst - 1];
+            o << "    if (base != 0) {\n";
+            lane_phys(st, "lz");
+            o << "    C* __restrict__ dz = psi + (base + lz);\n";
+            o << "    C z; z.x = R(0); z.y = R(0);\n";
+            for (int j = 0; j < NR; j++) {
+                if (pp_st >= 0) {
+                    if ((j >> pp_st) & 1) continue;
+                    o << "    { F4 v; v.x = 0.f; v.y = 0.f; v.z = 0.f; v.w = 0.f; ";
+                    if (nt_st) o << "__builtin_nontemporal_store(v, (F4*)(dz + " << reg_off(st, j) << "ull)); }\n";
+                    else o << "*(F4*)(dz + " << reg_off(st, j) << "ull) = v; }\n";
+                    continue;
+                }
+                if (nt_st) o << "    st(dz + " << reg_off(st, j) << "ull, z);\n";
+                else o << "    dz[" << reg_off(st, j) << "ull] = z;\n";
+            }
+            o << "    return;\n    }\n";
+        }
         o << "    C";
         for (int j = 0; j < NR; j++) o << (j ? ", " : " ") << "r" << j;
         if (split)
@@ -680,12 +701,14 @@ extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gat
 {
     HQ_CHECK(n >= 1 && n <= 36, HQ_ERR_ARG, "hq_sharemem_source: num_qubits must be in [1, 36]");
     HQ_CHECK(precision == HQ_C128 || precision == HQ_C64, HQ_ERR_ARG, "hq_sharemem_source: bad precision");
-    HQ_CHECK(variant == 0 || variant == 1, HQ_ERR_ARG, "hq_sharemem_source: variant must be 0 or 1");
+    HQ_CHECK(variant >= 0 && variant <= 3, HQ_ERR_ARG, "hq_sharemem_source: variant must be in [0, 3]");
+    const bool whole_line = variant & 1, init = (variant & 2) != 0;
     hq::ShmPlan plan;
-    hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan, false,
-                                     variant == 1 ? 3 : 0);
+    hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan, init,
+                                     whole_line ? 3 : 0);
     if (st != HQ_OK) return st;
-    plan.nt = variant == 1;
+    plan.nt = whole_line;
+    plan.init = init;
     std::vector<double> cf;
     std::string src = hq::jit_source(plan, &cf);
     if (src.empty()) return HQ_ERR_UNSUPPORTED;

@@ -172,11 +172,12 @@ hq_status hq_program_is_specialized(hq_program p, int *yes);
  * source of its specialised kernel (extern "C" hq_group(C *psi, const R *cf),
  * one block of 2^thread_qubits threads per tile, 2^(n - tile) blocks).
  * coef receives the runtime coefficient array (cf), shape[0..2] the tile,
- * register and thread qubit counts.  variant 0 is the plain kernel, 1 the
- * whole-line non-temporal one (stages without gates added where the first
- * or last stage would access HBM in pieces shorter than 128 bytes), the two
- * kernels the "tune" option chooses between.  For inspection, caching and
- * CPU tests. */
+ * register and thread qubit counts.  variant bit 0: 0 is the plain kernel,
+ * 1 the whole-line non-temporal one (stages without gates added where the
+ * first or last stage would access HBM in pieces shorter than 128 bytes),
+ * the two kernels the "tune" option chooses between; bit 1: the kernel of a
+ * run's first group, which builds |0...0> instead of loading the state.
+ * For inspection, caching and CPU tests. */
 hq_status hq_sharemem_source(int num_qubits, int precision, const hq_gate *gates, int num_gates,
                              uint64_t shared_mask, int reg_qubits, int variant, char *buf, size_t cap,
                              size_t *len, double *coef, size_t coef_cap, size_t *num_coef, int *shape);

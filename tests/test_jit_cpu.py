@@ -23,7 +23,9 @@ CLANG = "/opt/rocm/llvm/bin/clang++"
     (10, 9, 31, 0, 0, 0), (11, 10, 32, 0, 3, 0), (12, 11, 33, 1, 0, 0), (11, 11, 34, 0, 5, 0),
     (14, 13, 35, 0, 4, 0), (15, 13, 36, 0, 5, 0), (14, 13, 37, 1, 5, 0),
     (11, 10, 32, 0, 3, 1), (14, 13, 35, 0, 4, 1), (15, 13, 36, 0, 5, 1), (14, 13, 37, 1, 5, 1),
-    (15, 14, 38, 1, 0, 0), (16, 14, 39, 1, 6, 1), (15, 14, 40, 1, 6, 1), (13, 12, 41, 1, 4, 1)])
+    (15, 14, 38, 1, 0, 0), (16, 14, 39, 1, 6, 1), (15, 14, 40, 1, 6, 1), (13, 12, 41, 1, 4, 1),
+    # first group of a run (bit 1): builds |0...0>, other tiles only store zeros
+    (15, 13, 42, 0, 5, 2), (15, 13, 43, 0, 5, 3), (16, 14, 44, 1, 6, 3)])
 def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, variant):
     rs = np.random.default_rng(seed)
     rest = rs.choice(np.arange(3, n), k - 3, replace=False)
@@ -33,8 +35,9 @@ def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, var
     g = _group_for_mask(n, mask, 90, seed)
     src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, g, mask, prec, rb, variant)
     assert tile == k and (rb == 0 or rbits == rb)
-    if variant == 1:
-        assert "__builtin_nontemporal_load" in src and "__builtin_nontemporal_store" in src
+    if variant & 1:
+        assert "__builtin_nontemporal_store" in src
+        assert "__builtin_nontemporal_load" in src or variant & 2
     # non-temporal builtins and vector types (16-byte complex64 pairs) are
     # clang extensions: the host clang of ROCm runs them
     cxx = CLANG if os.path.exists(CLANG) else "g++"
@@ -52,6 +55,10 @@ def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, var
     subprocess.run([str(exe), str(n), str(k), str(1 << tbits), str(tmp_path / "in.bin"),
                     str(tmp_path / "coef.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
     got = np.fromfile(tmp_path / "out.bin", dtype=dt)
+    if variant & 2:
+        psi0 = np.zeros(1 << n, dtype=np.complex128)
+        psi0[0] = 1
+        assert "if (base != 0)" in src
     want = oracle.run_plain(g, n, psi0)
     assert rel_err(got, want) <= (1e-13 if prec == 0 else 2e-6)
 
