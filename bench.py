@@ -183,24 +183,31 @@ def main():
     bytes_per_pass = 2.0 * (1 << n) * amp_bytes
     # the first group of a run builds |0...0> in registers: it only writes the state
     init_fused = a.mode != "serial" and bool(sched) and sched[0]["backend"] == hq.HQ_BACKEND_PERGATE and n >= 7
+    # algorithmic bytes of each OShareMem pass: the whole state read and
+    # written, except the first (write only) and the early passes that skip
+    # tiles known to be zero (a run starts from |0...0>)
+    live = [g.get("live_fraction", 1.0) for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE]
+    pass_units = [f * (0.5 if (i == 0 and init_fused) else 1.0) for i, f in enumerate(live)]
     roof = None
     if shm:
         avg_ms = sum(shm) / len(shm)
         per_step = len(shm) // a.steps
-        alg_bytes = bytes_per_pass * (per_step - (0.5 if init_fused else 0.0)) / per_step
+        alg_bytes = bytes_per_pass * (sum(pass_units) if len(pass_units) == per_step else
+                                      per_step - (0.5 if init_fused else 0.0)) / per_step
         achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
         traffic = _traffic_from_profiles(n)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "hq_group" if jit_on else "k_sharemem", "avg_launch_ms": round(avg_ms, 4),
-                "launches_per_step": len(shm) // a.steps}
+                "launches_per_step": len(shm) // a.steps,
+                "pass_units_per_step": round(sum(pass_units), 4)}
     kinds = {}
     for k, ms in launches:
         kinds[k] = kinds.get(k, 0.0) + ms
     groups_pg = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE)
     groups_blas = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_BLAS)
     # whole-step roofline: every group needs one HBM pass (TransMM two)
-    t_ideal = (groups_pg + 2 * groups_blas - (0.5 if init_fused else 0.0)) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
+    t_ideal = (sum(pass_units) + 2 * groups_blas) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
 
     # TransMM groups: GEMM flops (8 K 2^n per group) over the dense kernels'
     # time, transposes as bytes moved (read + write of the state)

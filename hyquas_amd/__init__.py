@@ -117,6 +117,7 @@ def _declare(L):
         "hq_circuit_final_pos": (S, [c_void_p, POINTER(c_int)]),
         "hq_circuit_schedule": (S, [c_void_p, POINTER(c_int), c_int, POINTER(c_int), POINTER(c_int),
                                     POINTER(c_uint64), POINTER(c_int), POINTER(c_int)]),
+        "hq_circuit_schedule_live": (S, [c_void_p, c_int, POINTER(c_double), POINTER(c_int)]),
         "hq_circuit_schedule_order": (S, [c_void_p, POINTER(c_int), c_int]),
         "hq_circuit_last_times": (S, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
                                       POINTER(c_double)]),
@@ -613,8 +614,11 @@ class Circuit:
         be, ng, mq, st = (c_int * max(1, k))(), (c_int * max(1, k))(), (c_int * max(1, k))(), (c_int * max(1, k))()
         rel = (c_uint64 * max(1, k))()
         _check(lib().hq_circuit_schedule(self.handle, byref(n), k, be, ng, rel, mq, st))
-        return [dict(backend=be[i], num_gates=ng[i], related=rel[i], mat_qubit=mq[i], num_stages=st[i])
-                for i in range(k)]
+        live = (c_double * max(1, k))()
+        m = c_int()
+        _check(lib().hq_circuit_schedule_live(self.handle, k, live, byref(m)))
+        return [dict(backend=be[i], num_gates=ng[i], related=rel[i], mat_qubit=mq[i], num_stages=st[i],
+                     live_fraction=live[i]) for i in range(k)]
 
     def schedule_order(self):
         cnt = c_int()

@@ -123,6 +123,10 @@ struct ShmHeader {
     uint32_t num_stages;
     uint32_t num_ops;
     uint32_t pad;
+    // tiles whose block address has a bit in zero_mask hold only zeros before
+    // the pass (the run started from |0...0> and no earlier group reached
+    // those qubits) and keep them: the pass skips them
+    uint64_t zero_mask;
     // followed by ShmStage[num_stages], ShmOp[num_ops] (8-byte aligned)
 };
 

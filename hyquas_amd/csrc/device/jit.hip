@@ -378,6 +378,10 @@ struct Emitter {
             o << "    base |= ((b >> " << j << ") & " << hex64((1ull << len) - 1) << ") << " << p << ";\n";
             j += len - 1;
         }
+        if (P.hdr.zero_mask) {
+            // tiles known to be zero stay zero: nothing to load or store
+            o << "    if (base & " << hex64(P.hdr.zero_mask) << ") return;\n";
+        }
         if (P.init && P.hdr.num_block_bits > 0) {
             // first group of a run: every tile but the one holding |0...0>
             // starts and (the group's gates being linear) stays zero

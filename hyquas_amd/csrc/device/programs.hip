@@ -268,7 +268,8 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
     for (int v = 0; v < variants; v++) {
         for (size_t i = 0; i < reqs.size(); i++) {
             hq_status st = build_sharemem_program(s, reqs[i].gates.data(), (int)reqs[i].gates.size(), reqs[i].mask,
-                                                  reg_qubits, &built[v][i], reqs[i].init, tune ? v : only);
+                                                  reg_qubits, &built[v][i], reqs[i].init, tune ? v : only,
+                                                  reqs[i].zero_mask);
             if (st != HQ_OK) { destroy_all(); return st; }
         }
     }

@@ -404,7 +404,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
 }
 
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
-                                 hq_program* out, bool init, int variant)
+                                 hq_program* out, bool init, int variant, uint64_t zero_mask)
 {
     HQ_CHECK(s && out, HQ_ERR_ARG, "sharemem: null argument");
     *out = nullptr;
@@ -414,6 +414,7 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
     if (st != HQ_OK) return st;
     plan.init = init;
     plan.nt = variant == 1;
+    plan.hdr.zero_mask = init ? 0 : zero_mask;
     const auto& dst = plan.stages;
     const auto& ops = plan.ops;
     size_t bytes = sizeof(ShmHeader) + dst.size() * sizeof(ShmStage) + ops.size() * sizeof(ShmOp);

@@ -142,6 +142,8 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
     const uint32_t nbb = H->num_block_bits;
     for (uint32_t j = 0; j < nbb; j++)
         if ((blk >> j) & 1) base |= H->block_phys[j];
+    // a tile known to be zero stays zero (uniform over the block)
+    if (!(mode & TILE_READ) && (base & H->zero_mask)) return;
 
     uint32_t col[K];
 #pragma unroll

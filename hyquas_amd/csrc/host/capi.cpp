@@ -341,6 +341,19 @@ hq_status hq_circuit_schedule(hq_circuit c, int* num_groups, int cap, int* backe
     return HQ_OK;
 }
 
+hq_status hq_circuit_schedule_live(hq_circuit c, int cap, double* fraction, int* num_groups)
+{
+    HQ_CARG(c && num_groups, "hq_circuit_schedule_live: null argument");
+    int k = 0;
+    for (const auto& lg : c->c->getSchedule().localGroups)
+        for (const auto& gg : lg.fullGroups) {
+            if (fraction && k < cap) fraction[k] = gg.liveFraction;
+            k++;
+        }
+    *num_groups = k;
+    return HQ_OK;
+}
+
 hq_status hq_circuit_schedule_order(hq_circuit c, int* order, int cap)
 {
     HQ_CARG(c && order, "hq_circuit_schedule_order: null argument");

@@ -10,6 +10,7 @@
 #include "executor.h"
 
 #include <algorithm>
+#include <cmath>
 
 #include "hq_internal.h"
 
@@ -34,6 +35,14 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
     std::vector<GateGroup*> owners;
     bool first = true;
     sch.initFused = false;
+    // a run starts from |0...0>: until a group has had a qubit in its tile,
+    // every amplitude with that qubit set is zero, so tiles whose block
+    // address sets such a qubit are zero before the group and stay zero.
+    // `support` holds the physical qubits some earlier tile covered; it is
+    // tracked until the first TransMM group (its permutation moves qubits).
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    uint64_t support = 0;
+    bool track = true;
     for (auto& lg : sch.localGroups) {
         for (auto& gg : lg.fullGroups) {
             const bool isFirst = first;
@@ -63,10 +72,21 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                     // the run starts from |0...0>: the first pass builds it in registers
                     r.init = isFirst;
                     sch.initFused = sch.initFused || isFirst;
+                    gg.liveFraction = 1.0;
+                    if (track && !isFirst) {
+                        r.zero_mask = all & ~support;
+                        const int dead = __builtin_popcountll((unsigned long long)(r.zero_mask & ~r.mask));
+                        gg.liveFraction = std::ldexp(1.0, -dead);
+                    }
+                    support |= r.mask;
                     reqs.push_back(std::move(r));
                     owners.push_back(&gg);
+                } else {
+                    track = false;
                 }
             } else if (gg.backend == Backend::BLAS) {
+                track = false;
+                gg.liveFraction = 1.0;
                 gg.initCPUMatrix();
                 const size_t K = (size_t)1 << gg.matQubit;
                 std::vector<double> m(2 * K * K);

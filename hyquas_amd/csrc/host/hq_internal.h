@@ -15,7 +15,8 @@ void set_error(const std::string& msg);
 // HBM stages (added stages without gates where needed) with non-temporal
 // loads and stores in the specialised kernel.
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask,
-                                 int reg_qubits, hq_program* out, bool init = false, int variant = 0);
+                                 int reg_qubits, hq_program* out, bool init = false, int variant = 0,
+                                 uint64_t zero_mask = 0);
 
 // Many OShareMem programs at once; with jit, every group also gets a
 // structure-specialised kernel, compiled in parallel.  tune_mode (jit only):
@@ -26,6 +27,7 @@ struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;
     bool init = false;   // first group of a run: build |0...0> instead of loading the state
+    uint64_t zero_mask = 0;  // physical qubits still at |0> before this group (their tiles are skipped)
 };
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
                                   std::vector<hq_program>* out, int tune_mode = 0);

@@ -31,6 +31,7 @@ struct GateGroup {
     std::vector<std::complex<double>> matrix;  // K x K column-major (BLAS)
     hq_program program = nullptr; // device-resident program (owned)
     int numStages = 0;            // OShareMem register stages
+    double liveFraction = 1.0;    // share of the state's tiles the pass reads and writes (the others are known zero)
 
     GateGroup() = default;
     GateGroup(GateGroup&& o) noexcept;

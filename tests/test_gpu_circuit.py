@@ -102,6 +102,27 @@ def test_tuned_kernels_equal_plain(gpu, prec, mode):
         assert rel_err(states[1], want) <= 1e-10
 
 
+@pytest.mark.parametrize("jit", [0, 1])
+def test_zero_tiles_skipped(gpu, jit):
+    # early passes skip the tiles that are still zero (the run starts from
+    # |0...0>); the state must equal the per-gate oracle's
+    n = 22
+    c = gpu.Circuit(n).add_random_layers(10, 21)
+    c.set_option("jit", jit)
+    c.set_option("mode", gpu.HQ_MODE_GROUP)
+    c.compile()
+    live = [g["live_fraction"] for g in c.schedule()]
+    assert live[0] == 1.0 and live[-1] == 1.0
+    assert 0 < live[1] < 1.0
+    assert all(0 < f <= 1.0 for f in live)
+    c.run(copy_back=False, destroy=False)
+    want = oracle.run_plain(to_oracle(random_layers(n, 10, 21)), n)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+    # a second run over the state the first one left gives the same result
+    c.run(copy_back=False, destroy=False)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+
+
 def test_random_layers_full_vector(gpu):
     n = 18
     c = gpu.Circuit(n).add_random_layers(12, 3)
