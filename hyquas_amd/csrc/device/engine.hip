@@ -417,6 +417,11 @@ static hq_status compile_job(const std::vector<Gate>& gates, int n, int precisio
     p.coalesce = std::min(o.coalesce, k);
     Schedule sch = compileSchedule(n, gates, p);
     State ident(n);
+    // the first pass builds |0...0>; until a tile has covered a qubit, the
+    // tiles whose block address sets it are zero and later passes skip them
+    const uint64_t all = (1ull << n) - 1;
+    uint64_t support = 0;
+    bool first = true;
     for (auto& lg : sch.localGroups) {
         for (auto& gg : lg.fullGroups) {
             HQ_CHECK(gg.backend == Backend::PerGate, HQ_ERR_ARG, "engine: unexpected TransMM group");
@@ -427,6 +432,9 @@ static hq_status compile_job(const std::vector<Gate>& gates, int n, int precisio
             ShmPlan plan;
             hq_status st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), (uint64_t)rel, 0, &plan);
             if (st != HQ_OK) return st;
+            if (!first) plan.hdr.zero_mask = all & ~support;
+            support |= (uint64_t)rel;
+            first = false;
             append_blob(jp, plan);
         }
     }
