@@ -417,8 +417,9 @@ static hq_status compile_job(const std::vector<Gate>& gates, int n, int precisio
     p.coalesce = std::min(o.coalesce, k);
     Schedule sch = compileSchedule(n, gates, p);
     State ident(n);
-    // the first pass builds |0...0>; until a tile has covered a qubit, the
-    // tiles whose block address sets it are zero and later passes skip them
+    // the first pass builds |0...0>; until a non-diagonal gate has targeted
+    // a qubit, the tiles whose block address sets it are zero and later
+    // passes skip them
     const uint64_t all = (1ull << n) - 1;
     uint64_t support = 0;
     bool first = true;
@@ -433,7 +434,8 @@ static hq_status compile_job(const std::vector<Gate>& gates, int n, int precisio
             hq_status st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), (uint64_t)rel, 0, &plan);
             if (st != HQ_OK) return st;
             if (!first) plan.hdr.zero_mask = all & ~support;
-            support |= (uint64_t)rel;
+            for (const Gate& g : gg.gates)
+                if (!g.isDiagonal()) support |= 1ull << g.targetQubit;
             first = false;
             append_blob(jp, plan);
         }

@@ -35,11 +35,12 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
     std::vector<GateGroup*> owners;
     bool first = true;
     sch.initFused = false;
-    // a run starts from |0...0>: until a group has had a qubit in its tile,
-    // every amplitude with that qubit set is zero, so tiles whose block
-    // address sets such a qubit are zero before the group and stay zero.
-    // `support` holds the physical qubits some earlier tile covered; it is
-    // tracked until the first TransMM group (its permutation moves qubits).
+    // a run starts from |0...0>: until a non-diagonal gate has targeted a
+    // qubit (diagonal gates and controls leave a |0> qubit at |0>), every
+    // amplitude with that qubit set is zero, so tiles whose block address
+    // sets such a qubit are zero before the group and stay zero.  `support`
+    // holds the physical qubits some earlier non-diagonal gate targeted; it
+    // is tracked until the first TransMM group (its permutation moves qubits).
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     uint64_t support = 0;
     bool track = true;
@@ -78,7 +79,8 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                         const int dead = __builtin_popcountll((unsigned long long)(r.zero_mask & ~r.mask));
                         gg.liveFraction = std::ldexp(1.0, -dead);
                     }
-                    support |= r.mask;
+                    for (const Gate& g : gg.gates)
+                        if (!g.isDiagonal()) support |= 1ull << st.pos[g.targetQubit];
                     reqs.push_back(std::move(r));
                     owners.push_back(&gg);
                 } else {
