@@ -439,7 +439,10 @@ def main_precision(a, hq):
         _progress(f"precision: {n}q complex64 {sec:.4f} s per simulation")
         shm = [ms for k, ms in launches if k == hq.KIND_SHAREMEM]
         per_step = len(shm) // steps
-        alg = 2.0 * (1 << n) * 8 * (per_step - 0.5) / per_step
+        # live share of each pass (first: write only; early passes skip zero tiles)
+        live = [g["live_fraction"] for g in c.schedule() if g["backend"] == hq.HQ_BACKEND_PERGATE]
+        units = sum(f * (0.5 if i == 0 else 1.0) for i, f in enumerate(live))
+        alg = 2.0 * (1 << n) * 8 * (units if len(live) == per_step else per_step - 0.5) / per_step
         avg = sum(shm) / len(shm)
         norm = c.state_view(hq.HQ_C64).norm2()
         out_runs[f"{n}q_c64"] = {"s_per_sim": round(sec, 4), "groups": per_step, "compile_s": round(comp, 2),
