@@ -166,6 +166,22 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
         uint32_t b = ST[0].thr_bits[i];
         if ((tid >> i) & 1) { et |= 1u << b; pt |= H->tile_phys[b]; }
     }
+    if (mode == TILE_INIT && base != 0) {
+        // first pass of a run: a tile other than the origin's starts at zero
+        // and stays zero under the group's linear gates
+        cplx<R>* __restrict__ dst = psi + (base | pt);
+        cplx<R> z;
+        z.x = R(0);
+        z.y = R(0);
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            uint64_t off = 0;
+#pragma unroll
+            for (int i = 0; i < RB; i++) if ((j >> i) & 1) off |= rphys[i];
+            dst[off] = z;
+        }
+        return;
+    }
     if (mode & TILE_INIT) {
         const bool origin = (base | pt) == 0;
 #pragma unroll
