@@ -22,7 +22,6 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhyquas_amd.so")
 CLI = os.path.join(HERE, "hyquas_main")
-PROBES = os.path.join(CSRC, "probes")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -85,10 +84,6 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     _run([HIPCC, "--offload-arch=" + ARCH, "-o", CLI, _obj(main_src), "-L" + HERE, "-lhyquas_amd",
           "-Wl,-rpath,$ORIGIN", "-lgomp"], verbose)
-    # stand-alone measurement probes (not part of the library)
-    for f in sorted(os.listdir(PROBES)):
-        if f.endswith(".hip"):
-            _run([HIPCC] + HIP_FLAGS + [os.path.join(PROBES, f), "-o", os.path.join(HERE, f[:-4])], verbose)
     return LIB
 
 
