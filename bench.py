@@ -181,19 +181,16 @@ def main():
 
     shm = [ms for k, ms in launches if k == hq.KIND_SHAREMEM]
     bytes_per_pass = 2.0 * (1 << n) * amp_bytes
-    # the first group of a run builds |0...0> in registers: it only writes the state
-    init_fused = a.mode != "serial" and bool(sched) and sched[0]["backend"] == hq.HQ_BACKEND_PERGATE and n >= 7
-    # algorithmic bytes of each OShareMem pass: the whole state read and
-    # written, except the first (write only) and the early passes that skip
-    # tiles known to be zero (a run starts from |0...0>)
-    live = [g.get("live_fraction", 1.0) for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE]
-    pass_units = [f * (0.5 if (i == 0 and init_fused) else 1.0) for i, f in enumerate(live)]
+    # algorithmic bytes of each OShareMem pass: the share of a full read +
+    # write of the state it moves (the library reports it: 0.5 for the first
+    # pass, which builds |0...0> in registers and only writes; the live tiles'
+    # share for the early passes that skip tiles known to be zero)
+    pass_units = [g["live_fraction"] for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE]
     roof = None
     if shm:
         avg_ms = sum(shm) / len(shm)
         per_step = len(shm) // a.steps
-        alg_bytes = bytes_per_pass * (sum(pass_units) if len(pass_units) == per_step else
-                                      per_step - (0.5 if init_fused else 0.0)) / per_step
+        alg_bytes = bytes_per_pass * (sum(pass_units) if len(pass_units) == per_step else per_step) / per_step
         achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
         traffic = _traffic_from_profiles(n)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -441,8 +438,7 @@ def main_precision(a, hq):
         per_step = len(shm) // steps
         # live share of each pass (first: write only; early passes skip zero tiles)
         live = [g["live_fraction"] for g in c.schedule() if g["backend"] == hq.HQ_BACKEND_PERGATE]
-        units = sum(f * (0.5 if i == 0 else 1.0) for i, f in enumerate(live))
-        alg = 2.0 * (1 << n) * 8 * (units if len(live) == per_step else per_step - 0.5) / per_step
+        alg = 2.0 * (1 << n) * 8 * (sum(live) if len(live) == per_step else per_step) / per_step
         avg = sum(shm) / len(shm)
         norm = c.state_view(hq.HQ_C64).norm2()
         out_runs[f"{n}q_c64"] = {"s_per_sim": round(sec, 4), "groups": per_step, "compile_s": round(comp, 2),

@@ -73,7 +73,9 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                     // the run starts from |0...0>: the first pass builds it in registers
                     r.init = isFirst;
                     sch.initFused = sch.initFused || isFirst;
-                    gg.liveFraction = 1.0;
+                    // share of a full read + write of the state the pass moves:
+                    // the first pass only writes (its non-origin tiles store zeros)
+                    gg.liveFraction = isFirst ? 0.5 : 1.0;
                     if (track && !isFirst) {
                         r.zero_mask = all & ~support;
                         const int dead = __builtin_popcountll((unsigned long long)(r.zero_mask & ~r.mask));

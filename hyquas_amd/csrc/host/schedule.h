@@ -31,7 +31,7 @@ struct GateGroup {
     std::vector<std::complex<double>> matrix;  // K x K column-major (BLAS)
     hq_program program = nullptr; // device-resident program (owned)
     int numStages = 0;            // OShareMem register stages
-    double liveFraction = 1.0;    // share of the state's tiles the pass reads and writes (the others are known zero)
+    double liveFraction = 1.0;    // share of a full read + write of the state the pass moves (0.5: write only; <1: zero tiles skipped)
 
     GateGroup() = default;
     GateGroup(GateGroup&& o) noexcept;

@@ -306,10 +306,12 @@ hq_status hq_circuit_schedule(hq_circuit c, int *num_groups, int cap, int *backe
                               int *num_gates, uint64_t *related, int *mat_qubit,
                               int *num_stages);
 
-/* Share of the state's tiles each group's pass reads and writes, in schedule
- * order (compiled circuits): a run starts from |0...0>, so until some tile
- * has covered a qubit, tiles whose block address sets that qubit hold zeros
- * and the early OShareMem passes skip them (1 for every other pass). */
+/* Share of one full read + write of the state that each group's pass moves,
+ * in schedule order (compiled circuits).  A run starts from |0...0>: the first
+ * OShareMem pass builds it in registers and only writes (0.5); until some
+ * tile has covered a qubit, tiles whose block address sets that qubit hold
+ * zeros and the next passes skip them (the live tiles' share); 1 for every
+ * other pass. */
 hq_status hq_circuit_schedule_live(hq_circuit c, int cap, double *fraction, int *num_groups);
 
 /* Gate order of the schedule: order[i] = index (in addGate order) of the

@@ -105,11 +105,13 @@ def zero_state(n):
     return psi
 
 
-def run_plain(gates, n, psi=None, threads=0):
-    """Plain per-gate FP64 simulation (kernelSimple.cu semantics)."""
+def run_plain(gates, n, psi=None, threads=0, copy=True):
+    """Plain per-gate FP64 simulation (kernelSimple.cu semantics).  With
+    copy=False a contiguous complex128 `psi` is updated in place (large states)."""
     if psi is None:
         psi = zero_state(n)
-    psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+    if copy or psi.dtype != np.complex128 or not psi.flags.c_contiguous:
+        psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
     arr, p = _gp(gates)
     rc = lib().ora_run_plain(_dp(psi.view(np.float64)), n, p, len(arr), threads)
     if rc:

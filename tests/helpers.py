@@ -220,3 +220,38 @@ def bv_spec(n, secret):
 
 def ghz_spec(n):
     return [(H, 0, -1, ())] + [(CNOT, q + 1, q, ()) for q in range(n - 1)]
+
+
+def dense_block(qubits, m, seed):
+    """m gates on a few qubits: 30 % CNOT between two of them, else H, RX,
+    RY, RZ or T on one (numpy PCG64 draws)."""
+    rs = np.random.default_rng(seed)
+    out = []
+    for _ in range(m):
+        if rs.random() < 0.3:
+            a, b = rs.choice(qubits, 2, replace=False)
+            out.append((CNOT, int(b), int(a), ()))
+        else:
+            t = int(rs.choice([H, RX, RY, RZ, T]))
+            q = int(rs.choice(qubits))
+            out.append((t, q, -1, (float(rs.uniform(0, 2 * math.pi)),) if t in (RX, RY, RZ) else ()))
+    return out
+
+
+def blocks_and_layers(n, m=100):
+    """Random layers over all n qubits alternating with deep blocks on six
+    qubits: a circuit whose gates crowd onto few qubits in places, where a
+    dense TransMM group pays off beside OShareMem groups."""
+    assert n >= 20
+    return (random_layers(n, 1, 1) + dense_block([2, 6, 9, 12, 15, 17], m, 2) + random_layers(n, 2, 3) +
+            dense_block([0, 4, 7, 11, 16, 19], m, 4) + random_layers(n, 2, 5) +
+            dense_block([1, 3, 8, 13, 14, 18], m, 6) + random_layers(n, 1, 7))
+
+
+def flat_cost_tables(blas_ms, gate_cost=512.0, cutt=0.0):
+    """A parameter file in the reference's <n>qubits.out format
+    (src/evaluator.cpp:60-103, param_type 1): every gate costs `gate_cost`,
+    BLAS_perf[log2 K] = blas_ms for every K, transpose cost `cutt`."""
+    rows = ["1"] + [str(gate_cost)] * 14 + [""] + [str(gate_cost)] * 7 + [""]
+    rows += [f"{1 << i} {blas_ms}" for i in range(10)] + ["", str(cutt)]
+    return "\n".join(rows) + "\n"

@@ -5,7 +5,9 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import inverse, random_layers, random_mixed, rel_err, to_oracle
+from helpers import (blocks_and_layers, flat_cost_tables, inverse, random_layers, random_mixed, rel_err,
+                     to_oracle)
+from test_oracle import MIX_CASES, MIX_FLAT_COST, MIX_SPEC
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +39,64 @@ def test_modes_match_oracle(gpu, n, mode):
     got = _logical_state(c, n)
     want = oracle.run_plain(to_oracle(spec), n)
     assert rel_err(got, want) <= 1e-10
+
+
+@pytest.mark.parametrize("n,jit,tune", [(20, 1, 0), (20, 1, 1), (20, 0, 0), (22, 1, 0)])
+def test_mixed_schedule_interleaves_transmm_and_sharemem(gpu, tmp_path, n, jit, tune):
+    """A schedule whose groups alternate between OShareMem and TransMM, chosen
+    per group by cost per gate (src/compiler.cpp:248-279) under tables that
+    make a dense group pay off where gates crowd onto six qubits: OShareMem
+    groups lowered through the layouts the TransMM permutations leave, zero-tile
+    tracking stopped at the first TransMM group, the logical readout after the
+    sequence; complex128 against the per-gate oracle at 1e-10."""
+    spec = blocks_and_layers(n)
+    f = tmp_path / "cost.out"
+    f.write_text(flat_cost_tables(0.03))
+    c = _circuit(gpu, n, spec)
+    c.set_option("mode", gpu.HQ_MODE_MIX)
+    c.set_option("jit", jit)
+    c.set_option("tune", tune)
+    c.load_cost_model(str(f))
+    c.compile()
+    sched = c.schedule()
+    kinds = "".join("B" if g["backend"] == gpu.HQ_BACKEND_BLAS else "P" for g in sched)
+    print(n, jit, tune, kinds, flush=True)
+    assert kinds.count("B") >= 2 and "BP" in kinds and "PB" in kinds
+    first_b = kinds.index("B")
+    assert all(g["live_fraction"] == 1.0 for g in sched[first_b + 1:])
+    if first_b > 0:
+        assert sched[0]["live_fraction"] == 0.5
+    assert sum(g["num_gates"] for g in sched) == len(spec)
+    want = oracle.run_plain(to_oracle(spec), n)
+    c.run(copy_back=False, destroy=False)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+    # a second run starts over from |0...0> (the layout of the first run's end
+    # does not leak into it)
+    c.run(copy_back=False, destroy=False)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+
+
+@pytest.mark.parametrize("case", sorted(MIX_CASES))
+def test_hand_traced_mix_schedules_run(gpu, tmp_path, case):
+    """The hand-traced MIX-mode partitions of the oracle's tests, run on the
+    GPU: a TransMM group, then OShareMem groups on the permuted layout."""
+    n = 12
+    c = _circuit(gpu, n, MIX_SPEC)
+    c.set_option("profile", 1)
+    c.set_option("mode", gpu.HQ_MODE_MIX)
+    if MIX_CASES[case]["blas"] is not None:
+        f = tmp_path / "12qubits.out"
+        blas = MIX_CASES[case]["blas"]
+        rows = ["1"] + [str(MIX_FLAT_COST)] * 14 + [""] + [str(MIX_FLAT_COST)] * 7 + [""]
+        rows += [f"{1 << i} {blas.get(i, 1.0)}" for i in range(10)] + ["", "0"]
+        f.write_text("\n".join(rows) + "\n")
+        c.load_cost_model(str(f))
+    c.compile()
+    assert [g["backend"] for g in c.schedule()] == [g[0] for g in MIX_CASES[case]["groups"]]
+    c.run(copy_back=False, destroy=False)
+    assert c.final_pos() == MIX_CASES[case]["perm"]
+    want = oracle.run_plain(to_oracle(MIX_SPEC), n)
+    assert rel_err(c.logical_state(), want) <= 1e-12
 
 
 def test_random_layers_20q_full_compare(gpu):
@@ -112,7 +172,7 @@ def test_zero_tiles_skipped(gpu, jit):
     c.set_option("mode", gpu.HQ_MODE_GROUP)
     c.compile()
     live = [g["live_fraction"] for g in c.schedule()]
-    assert live[0] == 1.0 and live[-1] == 1.0
+    assert live[0] == 0.5 and live[-1] == 1.0   # the first pass only writes
     assert 0 < live[1] < 1.0
     assert all(0 < f <= 1.0 for f in live)
     c.run(copy_back=False, destroy=False)
@@ -187,7 +247,7 @@ def test_c64_14_qubit_tiles(gpu, jit):
     assert np.max(np.abs(got - want)) <= 1e-5
 
 
-@pytest.mark.parametrize("n,layers,prec", [(30, 30, "c128"), (33, 24, "c64")])
+@pytest.mark.parametrize("n,layers,prec", [(30, 30, "c128"), (33, 24, "c64"), (34, 24, "c64")])
 def test_round_trip_at_full_size(gpu, n, layers, prec):
     """The headline circuits at their full sizes, followed by their inverse,
     must return |0...0>: a size-independent check of the whole path (beam

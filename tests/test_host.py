@@ -7,7 +7,8 @@ import pytest
 
 import oracle
 from helpers import (CCX, CNOT, H, RX, hea_layers, random_layers, random_mixed, rel_err, to_oracle)
-from test_oracle import QASM, NPAR, _check_partition_invariants, _execute_partition
+from test_oracle import (MIX_CASES, MIX_FLAT_COST, MIX_SPEC, NPAR, QASM, _check_partition_invariants,
+                         _execute_partition)
 
 ALL_TYPES = list(range(22)) + [23, 24, 25, 26, 27]
 CTRL = {0: 2, 1: 1, 2: 1, 3: 1, 4: 1, 5: 1, 6: 1, 7: 1}
@@ -170,6 +171,40 @@ def test_cost_model_reference_format(hq, tmp_path):
     c.load_cost_model(str(f))
     c.partition()
     assert all(g["backend"] == hq.HQ_BACKEND_PERGATE for g in c.schedule())
+
+
+def _reference_format(singles, ctrls, blas, cutt):
+    rows = ["1"] + [str(v) for v in singles] + [""] + [str(v) for v in ctrls] + [""]
+    rows += [f"{1 << i} {blas.get(i, 1.0)}" for i in range(10)] + ["", str(cutt)]
+    return "\n".join(rows) + "\n"
+
+
+@pytest.mark.parametrize("case", sorted(MIX_CASES))
+def test_partition_hand_traced_mix_mode_in_product(hq, tmp_path, case):
+    """The hand-traced MIX-mode partitions of the oracle's tests (backend
+    chosen per group by cost per gate, src/compiler.cpp:248-279): the product
+    with the reference's rules and the same tables, given as a reference-format
+    parameter file (or the built-in V100 tables), gives the same groups,
+    backends, matrix sizes and layout."""
+    want = MIX_CASES[case]
+    c = hq.Circuit(12)
+    for s in MIX_SPEC:
+        c.addGate(hq.Gate.make(s[0], -1, s[2], s[1], s[3]))
+    c.set_option("profile", 1)
+    c.set_option("mode", hq.HQ_MODE_MIX)
+    if want["blas"] is not None:
+        f = tmp_path / "12qubits.out"
+        f.write_text(_reference_format([MIX_FLAT_COST] * 14, [MIX_FLAT_COST] * 7, want["blas"], 0.0))
+        c.load_cost_model(str(f))
+    c.partition()
+    order = c.schedule_order()
+    got, k = [], 0
+    for g in c.schedule():
+        got.append((g["backend"], order[k:k + g["num_gates"]], g["related"],
+                    g["mat_qubit"] if g["backend"] == hq.HQ_BACKEND_BLAS else 0))
+        k += g["num_gates"]
+    assert got == want["groups"]
+    assert c.final_pos() == want["perm"]
 
 
 def test_cost_model_key_value_format(hq, tmp_path):

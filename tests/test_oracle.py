@@ -304,6 +304,89 @@ def test_partition_hand_traced_blas_mode(ora):
     assert flay == [6, 7, 2, 3, 4, 5, 0, 1]
 
 
+# ---- MIX mode: the per-group backend choice (compiler.cpp:248-279, evaluator.cpp:168-229) ----
+#
+# Circuit (12 qubits): H on 8, 9, 10, 11, CNOT(8 -> 9), CNOT(10 -> 11), then H
+# on 0..7 (gates 6..13).  Tile 10, coalesce 3, matrices of 4..7 qubits.
+#
+# Tables "flat": every gate costs 512 (512 / 1000 / 512 = 0.001 ms per gate),
+# overhead 2^(12-27) ms per group, BLAS_perf[4] = blas4, BLAS_perf[5] = blas5,
+# BLAS_perf[6] = BLAS_perf[7] = 1, cutt 0.
+#
+# Group 1, OShareMem candidate (related starts at {0, 1, 2}): chains q8 = q9 =
+# {0, 1, 4}, q10 = q11 = {2, 3, 5}, q0..q7 one H each.  The merge takes q8
+# (3 gates; q0..q2 are absorbed: their related sets lie inside {0,1,2,8,9}),
+# then q10 (q11 absorbed), then q3, q4, q5 (10 qubits); q6, q7 do not fit:
+# gates 0..11, eff = (12 * 0.001 + 2^-15) / 12 = 1.0025e-3.
+# BLAS candidates (nothing preset): ms = 4 takes q8 + q10 chains, gates 0..5
+# (eff blas4 / 6); ms = 5 adds H(0) (blas5 / 7); ms = 6, 7 add H(1), H(2)
+# (1 / 8, 1 / 9).
+PG, BL = oracle.PERGATE, oracle.BLAS
+MIX_SPEC = ([(H, q, -1, ()) for q in (8, 9, 10, 11)] + [(CNOT, 9, 8, ()), (CNOT, 11, 10, ())] +
+            [(H, q, -1, ()) for q in range(8)])
+MIX_CASES = {
+    # blas4 / 6 = 8.3e-4 wins group 1 (ms = 4, qubits 8..11 swap with 0..3:
+    # perm = pos = [8,9,10,11,4,5,6,7,0,1,2,3]).  Group 2: tile preset to the
+    # logical qubits at physical 0..2 = {8, 9, 10}; H(0)..H(6) fill it (eff
+    # (7e-3 + 2^-15) / 7 = 1.0044e-3) against BLAS ms = 4 on H(0)..H(3):
+    # 0.005 / 4 = 1.25e-3 -> OShareMem.  Group 3: H(7) alone (1.03e-3 < 0.005).
+    "blas4": dict(blas={4: 0.005, 5: 1.0},
+                  groups=[(BL, [0, 1, 2, 3, 4, 5], 0xF00, 4), (PG, [6, 7, 8, 9, 10, 11, 12], 0x77F, 0),
+                          (PG, [13], 0x780, 0)],
+                  perm=[8, 9, 10, 11, 4, 5, 6, 7, 0, 1, 2, 3]),
+    # blas5 / 7 = 7.86e-4 beats blas4 / 6 = 8.3e-4: ms = 5, gates 0..6 on
+    # qubits {0, 8, 9, 10, 11}; qubit 0 is already low, 1..4 swap with 8..11:
+    # perm = [0,8,9,10,11,5,6,7,1,2,3,4].  Group 2: preset {0, 8, 9}; H(1)..H(7)
+    # fill the tile exactly (1.0044e-3) against ms = 4: 1.25e-3, ms = 5:
+    # 0.0055 / 5 = 1.1e-3 -> OShareMem.
+    "blas5": dict(blas={4: 0.005, 5: 0.0055},
+                  groups=[(BL, [0, 1, 2, 3, 4, 5, 6], 0xF01, 5), (PG, [7, 8, 9, 10, 11, 12, 13], 0x3FF, 0)],
+                  perm=[0, 8, 9, 10, 11, 5, 6, 7, 1, 2, 3, 4]),
+    # the reference's built-in V100 tables: group 1 costs (10 * 352 + 2 * 195)
+    # / 512000 + 2^-15 = 7.67e-3 ms for 12 gates (6.4e-4 per gate) against
+    # (BLAS_perf[4] = 0) + 11.37 ms for 6 gates; group 2 = H(6), H(7)
+    "v100": dict(blas=None, groups=[(PG, list(range(12)), 0xF3F, 0), (PG, [12, 13], 0x0C7, 0)],
+                 perm=list(range(12))),
+}
+MIX_FLAT_COST = 512.0
+
+
+def mix_params(case):
+    p = oracle.partition_params(12, oracle.MODE_MIX)
+    blas = MIX_CASES[case]["blas"]
+    if blas is not None:
+        for t in range(22):
+            p.single_cost[t] = MIX_FLAT_COST
+            p.ctr_cost[t] = MIX_FLAT_COST
+        for i in range(64):
+            p.blas_cost[i] = 1.0
+        for k, v in blas.items():
+            p.blas_cost[k] = v
+        p.cutt_cost = 0.0
+    return p
+
+
+@pytest.mark.parametrize("case", sorted(MIX_CASES))
+def test_partition_hand_traced_mix_mode(ora, case):
+    want = MIX_CASES[case]
+    groups, fpos, flay = oracle.partition(to_oracle(MIX_SPEC), 12, mix_params(case))
+    assert [(g["backend"], g["gates"], g["related"], g["mat_qubit"] if g["backend"] == oracle.BLAS else 0)
+            for g in groups] == want["groups"]
+    blas = [g for g in groups if g["backend"] == oracle.BLAS]
+    if blas:
+        assert blas[0]["perm"] == want["perm"]
+    assert fpos == want["perm"] and flay == want["perm"]
+    # the mixed schedule, executed on the oracle, is the circuit
+    psi, pos = _execute_partition(MIX_SPEC, 12, groups)
+    assert pos == fpos
+    ref = oracle.run_plain(to_oracle(MIX_SPEC), 12)
+    idx = np.arange(1 << 12)
+    phys = np.zeros_like(idx)
+    for q in range(12):
+        phys |= ((idx >> q) & 1) << fpos[q]
+    assert rel_err(psi[phys], ref) <= 1e-12
+
+
 # ---- readout and parser ---------------------------------------------------------------
 
 def test_print_state_text(ora):
