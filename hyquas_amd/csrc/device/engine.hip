@@ -973,10 +973,10 @@ struct hq_batch_s {
     int launches = 0;
     ~hq_batch_s()
     {
-        if (e) {
-            e->batches.erase(this);
+        // only a batch that hq_batch_create registered counted its blocks
+        // into batch_bytes (a failed create frees its blocks without it)
+        if (e && e->batches.erase(this))
             for (auto& j : jobs) for (auto& b : j->blocks) e->batch_bytes -= 1ull << b.order;
-        }
         if (e && e->stream) (void)hipStreamSynchronize(e->stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);

@@ -536,6 +536,36 @@ bool read_file(const std::string& p, std::string* out)
 
 const char* kSrcHeader = "// hyquas_amd group kernel v1\n";
 
+// hiprtc options of every specialised kernel
+const char* const kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=fast", "-Xclang", "-target-feature",
+                                "-Xclang", "-packed-fp32-ops"};
+constexpr int kNumJitOpts = (int)(sizeof(kJitOpts) / sizeof(kJitOpts[0]));
+
+// What besides the source decides the code object: the options and the
+// hiprtc version.  Part of the disk cache key, so a flag or toolchain change
+// never serves a code object built under other settings.
+const std::string& build_tag()
+{
+    static const std::string tag = [] {
+        std::string t;
+        for (const char* o : kJitOpts) t += std::string(o) + " ";
+        int major = 0, minor = 0;
+        if (hiprtcVersion(&major, &minor) == HIPRTC_SUCCESS)
+            t += "hiprtc " + std::to_string(major) + "." + std::to_string(minor);
+        return t;
+    }();
+    return tag;
+}
+
+bool write_file(const std::string& p, const char* data, size_t n)
+{
+    std::ofstream f(p, std::ios::binary);
+    if (!f) return false;
+    f.write(data, (std::streamsize)n);
+    f.close();
+    return !f.fail();
+}
+
 hq_status compile_source(const std::string& src, std::vector<char>* code, std::string* err)
 {
     {
@@ -544,8 +574,9 @@ hq_status compile_source(const std::string& src, std::vector<char>* code, std::s
         if (it != mem_cache().end()) { *code = it->second; return HQ_OK; }
     }
     const std::string dir = cache_dir();
-    char key[40];
-    snprintf(key, sizeof key, "%016llx_%zu", (unsigned long long)fnv1a(src), src.size());
+    char key[64];
+    snprintf(key, sizeof key, "%016llx_%016llx_%zu", (unsigned long long)fnv1a(build_tag()),
+             (unsigned long long)fnv1a(src), src.size());
     if (!dir.empty()) {
         std::string s2, c2;
         if (read_file(dir + "/" + key + ".hip", &s2) && s2 == src && read_file(dir + "/" + key + ".co", &c2) &&
@@ -561,9 +592,7 @@ hq_status compile_source(const std::string& src, std::vector<char>* code, std::s
         *err = "hiprtcCreateProgram failed";
         return HQ_ERR_HIP;
     }
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=fast", "-Xclang", "-target-feature",
-                          "-Xclang", "-packed-fp32-ops"};
-    hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+    hiprtcResult rc = hiprtcCompileProgram(prog, kNumJitOpts, kJitOpts);
     if (rc != HIPRTC_SUCCESS) {
         size_t n = 0;
         hiprtcGetProgramLogSize(prog, &n);
@@ -579,18 +608,19 @@ hq_status compile_source(const std::string& src, std::vector<char>* code, std::s
     hiprtcGetCode(prog, code->data());
     hiprtcDestroyProgram(&prog);
     if (!dir.empty()) {
+        // write both files under temporary names, then rename; a failed or
+        // short write leaves nothing behind (the disk cache is best effort)
         mkdirs(dir);
         std::string tmp = dir + "/" + key + ".tmp" + std::to_string((unsigned long long)(uintptr_t)code);
-        {
-            std::ofstream f(tmp + ".co", std::ios::binary);
-            f.write(code->data(), (std::streamsize)code->size());
+        const bool ok = write_file(tmp + ".co", code->data(), code->size()) &&
+                        write_file(tmp + ".hip", src.data(), src.size());
+        if (ok && rename((tmp + ".co").c_str(), (dir + "/" + key + ".co").c_str()) == 0) {
+            if (rename((tmp + ".hip").c_str(), (dir + "/" + key + ".hip").c_str()) != 0)
+                (void)remove((tmp + ".hip").c_str());
+        } else {
+            (void)remove((tmp + ".co").c_str());
+            (void)remove((tmp + ".hip").c_str());
         }
-        {
-            std::ofstream f(tmp + ".hip", std::ios::binary);
-            f << src;
-        }
-        rename((tmp + ".co").c_str(), (dir + "/" + key + ".co").c_str());
-        rename((tmp + ".hip").c_str(), (dir + "/" + key + ".hip").c_str());
     }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     mem_cache()[src] = *code;

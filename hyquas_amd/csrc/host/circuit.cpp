@@ -421,3 +421,18 @@ void Logger::print()
     for (auto& s : log_lines()) printf("Logger: %s\n", s.c_str());
     log_lines().clear();
 }
+
+namespace MyGlobalVars {
+int numGPUs = 1;
+int localGPUs = 1;
+int bit = 0;
+
+void init()
+{
+    int count = 0;
+    hq_status st = hq_device_count(&count);
+    if (st != HQ_OK) throw hq::Error(st, hq_last_error());
+    numGPUs = localGPUs = 1;
+    bit = 0;
+}
+}  // namespace MyGlobalVars

@@ -9,7 +9,9 @@
 #include "compiler.h"
 #include "executor.h"
 #include "gate.h"
+#include "logger.h"
 #include "schedule.h"
+#include "utils.h"
 
 qreal zero_wrapper(qreal x);   // src/utils.cpp:77-84
 
@@ -89,15 +91,6 @@ private:
     hq_state state = nullptr;
     bool stateHasBuffer = false;
     std::vector<qComplex> result;
-};
-
-// Logger of the reference (src/logger.h:9-48): "Logger: <line>" at exit.
-// Off by default, like the reference's SHOW_SUMMARY build option.
-class Logger {
-public:
-    static void add(const char* format, ...);
-    static void print();
-    static void enable(bool on);
 };
 
 // Opaque handle of the C ABI (hq_circuit): owns one Circuit.

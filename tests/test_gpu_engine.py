@@ -146,6 +146,34 @@ def test_budget_admission(gpu):
     e2.close()
 
 
+def test_failed_batch_keeps_admission_accounting(gpu):
+    """A batch whose create fails partway (its second state no longer fits the
+    budget) must leave the engine's count of bytes held by live batches as it
+    was: a later job that fits the budget but not beside the live batch, with
+    nothing else running, is then failed instead of waiting for memory that
+    only the live batch holds."""
+    import time
+    hq = gpu
+    e = hq.Engine(hq.HQ_C128, 1 << 27)                        # one 128 MiB chunk
+    live = e.batch([_rl(hq, 21, 2, s) for s in range(2)])    # 2 x 32 MiB, stays live
+    with pytest.raises(hq.HQError):
+        e.batch([_rl(hq, 21, 2, s) for s in range(3)])
+    j = e.submit(_rl(hq, 22, 2, 0), hq.HQ_OUT_EXPECT)        # 64 MiB
+    t0 = time.perf_counter()
+    while e.step() > 0 and time.perf_counter() - t0 < 20:
+        time.sleep(0.01)
+    assert e.status(j) == hq.HQ_JOB_FAILED
+    live.run()
+    ex, _ = live.results()
+    assert np.all(np.abs(ex[:, 0] - 1) < 1e-12)
+    live.close()
+    # with the batch gone the same job fits
+    j2 = e.submit(_rl(hq, 22, 2, 0), hq.HQ_OUT_EXPECT)
+    e.drain()
+    assert e.status(j2) == hq.HQ_JOB_DONE
+    e.close()
+
+
 def test_c64_close_to_c128(gpu):
     hq = gpu
     res = {}

@@ -1,6 +1,6 @@
 """CPU check of the specialised OShareMem kernel generator: the HIP source the
 library emits for a group is compiled with g++ and executed with one host
-thread per GPU thread (tests/jit_harness.cpp), then compared with the FP64
+thread per GPU thread (tests/cpp/jit_harness.cpp), then compared with the FP64
 oracle.  No GPU involved."""
 import os
 import shutil
@@ -47,7 +47,7 @@ def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, var
     ksrc.write_text(src)
     exe = tmp_path / "harness"
     subprocess.run([cxx, "-O1", "-std=c++20", "-pthread", "-Wno-unknown-attributes", f'-DGROUP_SRC="{ksrc}"',
-                    os.path.join(HERE, "jit_harness.cpp"), "-o", str(exe)], check=True)
+                    os.path.join(HERE, "cpp", "jit_harness.cpp"), "-o", str(exe)], check=True)
     psi0 = _rand_state(n, seed)
     dt = np.complex128 if prec == 0 else np.complex64
     psi0.astype(dt).tofile(tmp_path / "in.bin")
