@@ -95,6 +95,8 @@ def _declare(L):
         "hq_measure_zero": (S, [c_void_p, c_int, POINTER(c_double)]),
         "hq_state_norm": (S, [c_void_p, POINTER(c_double)]),
         "hq_expect_z": (S, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+        "hq_state_has_buffer": (S, [c_void_p, POINTER(c_int)]),
+        "hq_circuit_use_state": (S, [c_void_p, c_void_p]),
         "hq_state_inner": (S, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_double)]),
         "hq_profile_enable": (S, [c_void_p, c_int]),
         "hq_profile_read": (S, [c_void_p, POINTER(c_int), POINTER(c_double), c_int, POINTER(c_int)]),
@@ -592,6 +594,13 @@ class Circuit:
         h = c_void_p()
         _check(lib().hq_circuit_state(self.handle, byref(h)))
         return h
+
+    def use_state(self, state: "State | None"):
+        """Run on `state` (owned by the caller, kept allocated across runs and
+        after close()) instead of a state of the circuit's own; None undoes it."""
+        _check(lib().hq_circuit_use_state(self.handle, state.handle if state is not None else None))
+        self._borrowed = state
+        return self
 
     def state_view(self, precision: int = HQ_C128) -> State:
         """The device state after run(destroy=False), physical order."""

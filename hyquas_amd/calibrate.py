@@ -13,9 +13,10 @@ Writes
                          rows, "K ms" rows for K = 1..512, transpose cost; the
                          rows are times of 512-gate circuits in microseconds,
                          like process.cpp:20-82), extended to complex64 with
-                         --c64 and to K = 128 by measurement (K > 128 is
-                         extrapolated linearly: the dense kernel stops at 7
-                         qubits).
+                         --c64; the dense rows K = 2..128 are measured, K = 1
+                         repeats K = 2 and K = 256, 512 are the K = 128 time
+                         scaled by K / 128 (the dense kernel stops at 7
+                         qubits, and the partitioner reads rows 4..7 only).
 
 Either file loads with ``Circuit.load_cost_model(path)``
 (``hq_circuit_load_cost_model``).  Prints one JSON summary line.
@@ -54,6 +55,7 @@ def run_length_table(n, precision, samples=4, seed=0, tile=RUN_TILE):
     qubits drawn at random above qubit L (median of `samples` draws).  The pass
     is a specialised kernel holding one diagonal gate (a kernel without gates
     loads and stores the same values, which the compiler removes)."""
+    tile = min(tile, n - 1)
     st = hq.State(n, precision)
     st.profile(True)
     b = 16 if precision == hq.HQ_C128 else 8
@@ -162,29 +164,39 @@ def reference_tables(n, precision, path):
         c.close()
         return us
 
-    lines = ["1"]
-    for t in singles:
-        gs = [hq.Gate.make(t, -1, -1, 1, tuple(rng.uniform(0, 2 * math.pi, npar.get(t, 0)))) for _ in range(512)]
-        lines.append(str(circuit_us(gs)))
-    lines.append("")
-    for t in ctrls:
-        gs = [hq.Gate.make(t, -1, 0, 2, tuple(rng.uniform(0, 2 * math.pi, npar.get(t, 0)))) for _ in range(512)]
-        lines.append(str(circuit_us(gs)))
-    rates, times, tms, _ = dense_and_transpose(n, precision)
-    lines.append("")
+    su = [circuit_us([hq.Gate.make(t, -1, -1, 1, tuple(rng.uniform(0, 2 * math.pi, npar.get(t, 0))))
+                      for _ in range(512)]) for t in singles]
+    cu = [circuit_us([hq.Gate.make(t, -1, 0, 2, tuple(rng.uniform(0, 2 * math.pi, npar.get(t, 0))))
+                      for _ in range(512)]) for t in ctrls]
+    _, times, tms, _ = dense_and_transpose(n, precision)
+    with open(path, "w") as f:
+        f.write(reference_table_text(su, cu, times, tms))
+
+
+def reference_table_text(singles_us, ctrls_us, dense_ms, transpose_ms):
+    """The reference's <n>qubits.out text (param_type 1, read by
+    src/evaluator.cpp:60-103): the 14 single-gate rows (U1, U2, U3, H, X, Y, Z,
+    S, SDG, T, TDG, RX, RY, RZ), a blank line, the 7 controlled-gate rows
+    (CNOT, CY, CZ, CRX, CRY, CU1, CRZ), the "K ms" rows for K = 1..512 and the
+    transpose cost in ms.  dense_ms maps K = 2..128 to measured times."""
+    assert len(singles_us) == 14 and len(ctrls_us) == 7
+    lines = ["1"] + [str(v) for v in singles_us] + [""] + [str(v) for v in ctrls_us] + [""]
     for i in range(10):
         K = 1 << i
-        if K in times:
-            ms = times[K]
-        elif K == 1:
-            ms = times[2]
-        else:
-            ms = times[128] * K / 128
+        ms = dense_ms[K] if K in dense_ms else dense_ms[2] if K == 1 else dense_ms[128] * K / 128
         lines.append(f"{K} {ms:.6f}")
-    lines.append("")
-    lines.append(f"{tms:.6f}")
-    with open(path, "w") as f:
-        f.write("\n".join(lines) + "\n")
+    lines += ["", f"{transpose_ms:.6f}"]
+    return "\n".join(lines) + "\n"
+
+
+def params_text(n, hbm_bw, valu64, valu32, mfma64, mfma32, launch, bw64, bw32):
+    """This build's model file ("key value" lines, read by CostModel::load)."""
+    out = [f"# MI355X cost model, calibrated at {n} qubits by hyquas_amd.calibrate",
+           f"hbm_bw {hbm_bw:.6e}", f"valu_rate {valu64:.6e}", f"valu_rate_f32 {valu32:.6e}",
+           f"mfma_f64 {mfma64:.6e}", f"mfma_f32 {mfma32:.6e}", f"launch_overhead {launch:.6e}"]
+    out += [f"shm_bw_run_{L} {b:.6e}" for L, b in enumerate(bw64)]
+    out += [f"shm_bw_run_f32_{L} {b:.6e}" for L, b in enumerate(bw32)]
+    return "\n".join(out) + "\n"
 
 
 def main(argv=None):
@@ -193,12 +205,13 @@ def main(argv=None):
     ap.add_argument("--out", default="calibration")
     ap.add_argument("--tables", default="", help="qubit counts for the reference-format tables, e.g. 22,24,26,28")
     ap.add_argument("--c64", action="store_true", help="reference-format tables in complex64")
+    ap.add_argument("--samples", type=int, default=4, help="random tiles per run length")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     n = a.qubits
     t0 = time.perf_counter()
-    bw64 = run_length_table(n, hq.HQ_C128)
-    bw32 = run_length_table(n, hq.HQ_C64)
+    bw64 = run_length_table(n, hq.HQ_C128, a.samples)
+    bw32 = run_length_table(n, hq.HQ_C64, a.samples)
     v64 = valu_rate(n, hq.HQ_C128)
     v32 = valu_rate(n, hq.HQ_C64)
     r64, _, t64, tbw = dense_and_transpose(n - 1, hq.HQ_C128)
@@ -206,15 +219,7 @@ def main(argv=None):
     lo = launch_overhead()
     params = os.path.join(a.out, "mi355x.params")
     with open(params, "w") as f:
-        f.write(f"# MI355X cost model, calibrated at {n} qubits by hyquas_amd.calibrate\n")
-        f.write(f"hbm_bw {tbw:.6e}\n")
-        f.write(f"valu_rate {v64:.6e}\nvalu_rate_f32 {v32:.6e}\n")
-        f.write(f"mfma_f64 {r64[64]:.6e}\nmfma_f32 {r32[128]:.6e}\n")
-        f.write(f"launch_overhead {lo:.6e}\n")
-        for L, b in enumerate(bw64):
-            f.write(f"shm_bw_run_{L} {b:.6e}\n")
-        for L, b in enumerate(bw32):
-            f.write(f"shm_bw_run_f32_{L} {b:.6e}\n")
+        f.write(params_text(n, tbw, v64, v32, r64[64], r32[128], lo, bw64, bw32))
     tables = []
     for m in [int(x) for x in a.tables.split(",") if x]:
         path = os.path.join(a.out, f"{m}qubits.out")

@@ -12,6 +12,19 @@
 
 namespace hq {
 
+// Global-memory pointers.  The batched kernel reads its program and state
+// pointers from an item table, which makes them generic (flat) pointers; the
+// compiler takes every flat load for a per-lane value, so the op dispatch
+// below would compile to divergent branches.  Casting to the global address
+// space keeps the program's fields uniform (scalar loads, scalar branches).
+// (The host pass of a HIP file sees the same templates: no address space there.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HQ_GLOBAL __attribute__((address_space(1)))
+#else
+#define HQ_GLOBAL
+#endif
+template <typename T> __device__ __forceinline__ HQ_GLOBAL T* to_global(T* p) { return (HQ_GLOBAL T*)p; }
+
 // tile_pass modes (bit set)
 constexpr uint32_t TILE_INIT = 1;     // start from |0...0> instead of loading the tile
 constexpr uint32_t TILE_NOSTORE = 2;  // do not write the tile back
@@ -59,16 +72,26 @@ __device__ __forceinline__ void reg_op(cplx<R> (&a)[1 << RB], int cls, uint32_t 
 }
 
 template <typename R, int RB>
-__device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const ShmOp* __restrict__ op, uint32_t tid,
+__device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const HQ_GLOBAL ShmOp* __restrict__ op, uint32_t tid,
                                          uint64_t blk)
 {
+    // Every branch below is uniform over the block (the op fields and the
+    // block index are): a lane whose thread-bit controls are not all set gets
+    // the identity's coefficients instead of leaving the op, so no branch
+    // depends on the lane (a per-lane exit made every later dispatch a
+    // divergent branch).  Thread-controlled ops only use classes that can
+    // express the identity (the planner turns SWAP into REAL and Y into RX).
     const uint64_t cblk = op->cblk;
     if ((blk & cblk) != cblk) return;                 // block control not set (uniform)
     const uint32_t cthr = op->cthr;
-    if ((tid & cthr) != cthr) return;                 // thread control not set (per lane)
+    const bool on = (tid & cthr) == cthr;
     R m[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) m[i] = (R)op->m[i];
+    for (int i = 0; i < 8; i++) {
+        const R id = (i == 0 || i == 6) ? R(1) : R(0);
+        const R v = (R)op->m[i];
+        m[i] = on ? v : id;
+    }
     const uint32_t creg = op->creg;
     const int cls = op->cls;
     const int tpos = op->tpos;
@@ -90,11 +113,10 @@ __device__ __forceinline__ void apply_op(cplx<R> (&a)[1 << RB], const ShmOp* __r
         return;
     }
     // diagonal gate whose target is a thread bit or a block bit: one factor
-    // per lane for every register amplitude
-    uint32_t hb = op->where == SHM_IN_THREAD ? (tid >> tpos) & 1u : (uint32_t)((blk >> tpos) & 1u);
-    if (cls == SHM_PHASE && !hb) return;
-    const R fr = hb ? m[6] : m[0];
-    const R fi = hb ? m[7] : m[1];
+    // per lane for every register amplitude (a phase leaves the |0> half alone)
+    const uint32_t hb = op->where == SHM_IN_THREAD ? (tid >> tpos) & 1u : (uint32_t)((blk >> tpos) & 1u);
+    const R fr = hb ? m[6] : (cls == SHM_PHASE ? R(1) : m[0]);
+    const R fi = hb ? m[7] : (cls == SHM_PHASE ? R(0) : m[1]);
 #pragma unroll
     for (int j = 0; j < (1 << RB); j++) {
         if ((j & creg) != creg) continue;
@@ -124,17 +146,21 @@ __device__ __forceinline__ void zsums(uint64_t idx, int n, double* z, double* zz
 // One OShareMem pass of tile `blk` of the group program `prog` over `psi`.
 // `mode` is uniform over the grid; the single-state kernel passes 0.
 template <typename R, int RB, int TB>
-__device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char* __restrict__ prog, uint64_t blk,
-                                          cplx<R>* __restrict__ lds, uint32_t mode, int n,
-                                          double* __restrict__ part)
+__device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi_any, const char* __restrict__ prog_any,
+                                          uint64_t blk, cplx<R>* __restrict__ lds, uint32_t mode, int n,
+                                          double* __restrict__ part_any)
 {
+    HQ_GLOBAL cplx<R>* __restrict__ psi = to_global(psi_any);
+    const HQ_GLOBAL char* __restrict__ prog = to_global(prog_any);
+    HQ_GLOBAL double* __restrict__ part = to_global(part_any);
     constexpr int NR = 1 << RB;
     constexpr int K = RB + TB;
 
-    const ShmHeader* __restrict__ H = (const ShmHeader*)prog;
-    const ShmStage* __restrict__ ST = (const ShmStage*)(prog + sizeof(ShmHeader));
+    const HQ_GLOBAL ShmHeader* __restrict__ H = (const HQ_GLOBAL ShmHeader*)prog;
+    const HQ_GLOBAL ShmStage* __restrict__ ST = (const HQ_GLOBAL ShmStage*)(prog + sizeof(ShmHeader));
     const uint32_t nst = H->num_stages;
-    const ShmOp* __restrict__ OP = (const ShmOp*)(prog + sizeof(ShmHeader) + nst * sizeof(ShmStage));
+    const HQ_GLOBAL ShmOp* __restrict__ OP =
+        (const HQ_GLOBAL ShmOp*)(prog + sizeof(ShmHeader) + nst * sizeof(ShmStage));
     const uint32_t tid = threadIdx.x;
 
     // block base address: pdep(blk, block bits)
@@ -169,7 +195,7 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
     if (mode == TILE_INIT && base != 0) {
         // first pass of a run: a tile other than the origin's starts at zero
         // and stays zero under the group's linear gates
-        cplx<R>* __restrict__ dst = psi + (base | pt);
+        HQ_GLOBAL cplx<R>* __restrict__ dst = psi + (base | pt);
         cplx<R> z;
         z.x = R(0);
         z.y = R(0);
@@ -190,7 +216,7 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
             a[j].y = R(0);
         }
     } else {
-        const cplx<R>* __restrict__ src = psi + (base | pt);
+        const HQ_GLOBAL cplx<R>* __restrict__ src = psi + (base | pt);
 #pragma unroll
         for (int j = 0; j < NR; j++) {
             uint64_t off = 0;
@@ -284,7 +310,7 @@ __device__ __forceinline__ void tile_pass(cplx<R>* __restrict__ psi, const char*
     }
 
     if (!(mode & TILE_NOSTORE)) {
-        cplx<R>* __restrict__ dst = psi + (base | pt);
+        HQ_GLOBAL cplx<R>* __restrict__ dst = psi + (base | pt);
 #pragma unroll
         for (int j = 0; j < NR; j++) {
             uint64_t off = 0;

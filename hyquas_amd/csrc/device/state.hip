@@ -273,6 +273,13 @@ hq_status hq_state_info(hq_state s, int* num_qubits, int* precision)
     return HQ_OK;
 }
 
+hq_status hq_state_has_buffer(hq_state s, int* has_buffer)
+{
+    HQ_CHECK(s && has_buffer, HQ_ERR_ARG, "hq_state_has_buffer: null argument");
+    *has_buffer = s->buf ? 1 : 0;
+    return HQ_OK;
+}
+
 hq_status hq_state_sync(hq_state s)
 {
     HQ_CHECK(s, HQ_ERR_ARG, "hq_state_sync: null state");

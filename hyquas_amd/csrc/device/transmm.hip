@@ -233,7 +233,14 @@ template <> __device__ __forceinline__ int mfma_row<float>(int lane, int v) { re
 template <> __device__ __forceinline__ int mfma_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
 // One wave per 16-row block of U (K/16 waves), NT columns per tile.
-template <typename R, int MQ, int NT>
+//
+// GAUSS: the complex product with three real MFMAs instead of four (Gauss's
+// method): with T1 = Ar Br, T2 = Ai Bi and T3 = (Ar + Ai)(Br + Bi) summed
+// over k, Re C = T1 - T2 and Im C = T3 - T1 - T2.  A quarter fewer matrix-core
+// operations for the same product; the sums Ar + Ai are formed once per
+// kernel, Br + Bi once per B element in LDS.  The extra rounding is one
+// addition per operand and one per output (FP64: ~1e-16 relative).
+template <typename R, int MQ, int NT, bool GAUSS>
 __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx<R>* __restrict__ in,
                                                                      cplx<R>* __restrict__ out,
                                                                      const R* __restrict__ ure,
@@ -254,12 +261,13 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
     const int kq = lane >> 4;          // 0..3
 
     // U fragments of this wave's row block: A[row][4q + kq]
-    R ar[KS], ai[KS];
+    R ar[KS], ai[KS], as[GAUSS ? KS : 1];
 #pragma unroll
     for (int q = 0; q < KS; q++) {
         int kk = 4 * q + kq;
         ar[q] = ure[kk * K + arow];     // column-major: U(row, col) at col*K + row
         ai[q] = uim[kk * K + arow];
+        if constexpr (GAUSS) as[q] = ar[q] + ai[q];
     }
 
     const uint64_t ntiles = ncols / NT;
@@ -273,11 +281,12 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
             bim[k * NTP + col] = v.y;
         }
         __syncthreads();
-        Acc cr[NCB], ci[NCB];
+        Acc cr[NCB], ci[NCB], cs[GAUSS ? NCB : 1];
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
             cr[c] = Acc{0, 0, 0, 0};
             ci[c] = Acc{0, 0, 0, 0};
+            if constexpr (GAUSS) cs[c] = Acc{0, 0, 0, 0};
         }
 #pragma unroll
         for (int q = 0; q < KS; q++) {
@@ -286,10 +295,24 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
             for (int c = 0; c < NCB; c++) {
                 R br = bre[kk * NTP + c * 16 + (lane & 15)];
                 R bi = bim[kk * NTP + c * 16 + (lane & 15)];
-                cr[c] = mfma16x16x4(ar[q], br, cr[c]);
-                cr[c] = mfma16x16x4(ai[q], -bi, cr[c]);
-                ci[c] = mfma16x16x4(ar[q], bi, ci[c]);
-                ci[c] = mfma16x16x4(ai[q], br, ci[c]);
+                if constexpr (GAUSS) {
+                    cr[c] = mfma16x16x4(ar[q], br, cr[c]);        // T1
+                    ci[c] = mfma16x16x4(ai[q], bi, ci[c]);        // T2
+                    cs[c] = mfma16x16x4(as[q], br + bi, cs[c]);   // T3
+                } else {
+                    cr[c] = mfma16x16x4(ar[q], br, cr[c]);
+                    cr[c] = mfma16x16x4(ai[q], -bi, cr[c]);
+                    ci[c] = mfma16x16x4(ar[q], bi, ci[c]);
+                    ci[c] = mfma16x16x4(ai[q], br, ci[c]);
+                }
+            }
+        }
+        if constexpr (GAUSS) {
+#pragma unroll
+            for (int c = 0; c < NCB; c++) {
+                const Acc t1 = cr[c], t2 = ci[c];
+                cr[c] = t1 - t2;
+                ci[c] = cs[c] - t1 - t2;
             }
         }
         __syncthreads();
@@ -317,14 +340,18 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
     }
 }
 
+// complex128 uses Gauss's three-product form (FP64 matrix cores are the
+// limit at K >= 64); complex64 keeps four products (FP32 matrix cores run at
+// twice the FP64 rate and Gauss's form adds rounding at single precision)
 template <typename R, int MQ, int NT>
 static hq_status launch_mfma(hq_state s, const R* ure, const R* uim, void* in, void* out)
 {
+    constexpr bool gauss = sizeof(R) == 8;
     constexpr int K = 1 << MQ;
     uint64_t ncols = s->amps / K;
     uint64_t ntiles = ncols / NT;
     uint64_t grid = std::min<uint64_t>(ntiles, 2048);
-    hipLaunchKernelGGL((k_dense_mfma<R, MQ, NT>), dim3((unsigned)grid), dim3(64 * (K / 16)), 0, s->stream,
+    hipLaunchKernelGGL((k_dense_mfma<R, MQ, NT, gauss>), dim3((unsigned)grid), dim3(64 * (K / 16)), 0, s->stream,
                        (const cplx<R>*)in, (cplx<R>*)out, ure, uim, ncols);
     HQ_HIP(hipGetLastError());
     return HQ_OK;
@@ -339,7 +366,10 @@ static hq_status launch_dense_t(hq_state s, const void* d_mat, int mq, void* in,
     // MFMA path needs whole column tiles
     if (mq == 4 && s->amps >= 16 * 64) return launch_mfma<R, 4, 64>(s, ure, uim, in, out);
     if (mq == 5 && s->amps >= 32 * 64) return launch_mfma<R, 5, 64>(s, ure, uim, in, out);
-    if (mq == 6 && s->amps >= 64 * 64) return launch_mfma<R, 6, 64>(s, ure, uim, in, out);
+    // complex128 (three products, three accumulators per column block) takes
+    // 32-column tiles at K = 64 to stay within 256 registers (two waves per SIMD)
+    constexpr int nt6 = sizeof(R) == 8 ? 32 : 64;
+    if (mq == 6 && s->amps >= 64 * nt6) return launch_mfma<R, 6, nt6>(s, ure, uim, in, out);
     if (mq == 7 && s->amps >= 128 * 32) return launch_mfma<R, 7, 32>(s, ure, uim, in, out);
     uint64_t blocks = std::min<uint64_t>((s->amps + 255) / 256, 1u << 20);
     hipLaunchKernelGGL(k_dense_simple<R>, dim3((unsigned)blocks), dim3(256), 0, s->stream, (const cplx<R>*)in,

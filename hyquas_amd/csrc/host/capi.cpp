@@ -308,6 +308,12 @@ hq_status hq_circuit_state(hq_circuit c, hq_state* out)
     return HQ_OK;
 }
 
+hq_status hq_circuit_use_state(hq_circuit c, hq_state s)
+{
+    HQ_CARG(c, "hq_circuit_use_state: null circuit");
+    return c->c->useState(s);
+}
+
 hq_status hq_circuit_final_pos(hq_circuit c, int* pos)
 {
     HQ_CARG(c && pos, "hq_circuit_final_pos: null argument");

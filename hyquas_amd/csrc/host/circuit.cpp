@@ -38,9 +38,32 @@ Circuit::~Circuit() { freeState(); }
 
 void Circuit::freeState()
 {
-    if (state) hq_state_destroy(state);
+    // a borrowed state belongs to the caller: only the reference is dropped
+    if (state && !stateBorrowed) hq_state_destroy(state);
     state = nullptr;
     stateHasBuffer = false;
+    stateBorrowed = false;
+}
+
+hq_status Circuit::useState(hq_state s)
+{
+    int hb = 0;
+    if (s) {
+        int n = 0, prec = 0;
+        hq_status rc = hq_state_info(s, &n, &prec);
+        if (rc == HQ_OK) rc = hq_state_has_buffer(s, &hb);
+        if (rc != HQ_OK) return rc;
+        if (n != numQubits) {
+            hq::set_error("use_state: the state has " + std::to_string(n) + " qubits, the circuit " +
+                          std::to_string(numQubits));
+            return HQ_ERR_ARG;
+        }
+    }
+    freeState();
+    state = s;
+    stateBorrowed = s != nullptr;
+    stateHasBuffer = hb != 0;
+    return HQ_OK;
 }
 
 bool Circuit::needsBuffer() const
@@ -54,6 +77,20 @@ bool Circuit::needsBuffer() const
 hq_status Circuit::ensureState()
 {
     bool buf = needsBuffer();
+    if (stateBorrowed) {
+        int n, prec;
+        hq_status rc = hq_state_info(state, &n, &prec);
+        if (rc != HQ_OK) return rc;
+        if (prec != params.precision) {
+            hq::set_error("the state given with use_state has another precision than the circuit");
+            return HQ_ERR_ARG;
+        }
+        if (buf && !stateHasBuffer) {
+            hq::set_error("the schedule has TransMM groups: the state given with use_state needs a work buffer");
+            return HQ_ERR_ARG;
+        }
+        return HQ_OK;
+    }
     if (state) {
         int n, prec;
         hq_state_info(state, &n, &prec);
@@ -285,7 +322,7 @@ hq_status Circuit::runChecked(bool copy_back, bool destroy, long long* us)
         }
         if (rc != HQ_OK) return rc;
     }
-    if (destroy) freeState();
+    if (destroy && !stateBorrowed) freeState();
     return HQ_OK;
 }
 

@@ -71,6 +71,9 @@ public:
     const std::vector<Gate>& getGates() const { return gates; }
     const Schedule& getSchedule() const { return schedule; }
     hq_state deviceState() const { return state; }
+    // Run on a caller-owned state (borrowed: never destroyed here); nullptr
+    // returns to states of the circuit's own.
+    hq_status useState(hq_state s);
     const std::vector<int>& finalPos() const { return schedule.finalState.pos; }
     double lastTimes[4] = {0, 0, 0, 0};   // ms by launch kind (profile on)
     std::vector<std::pair<int, double>> lastLaunches;
@@ -90,6 +93,7 @@ private:
     bool compiled = false;
     hq_state state = nullptr;
     bool stateHasBuffer = false;
+    bool stateBorrowed = false;
     std::vector<qComplex> result;
 };
 

@@ -130,6 +130,9 @@ hq_status hq_state_sync(hq_state s);
 /* Number of qubits / precision of a state. */
 hq_status hq_state_info(hq_state s, int *num_qubits, int *precision);
 
+/* Whether the state was created with a work buffer (TransMM groups need one). */
+hq_status hq_state_has_buffer(hq_state s, int *has_buffer);
+
 /* Plain method: one HBM pass per gate, in order (kernelExecSimple,
  * src/kernelSimple.cu:335-446).  Asynchronous. */
 hq_status hq_apply_gates_plain(hq_state s, const hq_gate *gates, int num_gates);
@@ -295,6 +298,15 @@ hq_status hq_circuit_print_state(hq_circuit c, int show_schedule, char *buf,
 /* Borrow the device state after run(destroy=0) for the device readouts
  * (physical indices: combine with hq_circuit_final_pos). */
 hq_status hq_circuit_state(hq_circuit c, hq_state *out);
+
+/* Run c on a state the caller owns (a resident state reused by circuit after
+ * circuit) instead of one that run() allocates: s must have c's qubit count
+ * and, at compile/run time, c's precision and a work buffer if the schedule
+ * has TransMM groups.  c borrows s: run(destroy=1) and hq_circuit_destroy
+ * leave it allocated, and s must outlive its use by c.  s = NULL returns c to
+ * states of its own.  No counterpart in the reference, whose Circuit
+ * allocates its state in run() (src/circuit.cpp:16-71). */
+hq_status hq_circuit_use_state(hq_circuit c, hq_state s);
 
 /* Final logical->physical map (schedule.finalState.pos), pos[n]. */
 hq_status hq_circuit_final_pos(hq_circuit c, int *pos);

@@ -90,5 +90,6 @@ def test_reference_style_driver_runs(gpu, tmp_path):
     psi = oracle.run_plain(to_oracle(driver_spec(n)), n)
     want = oracle.print_state(psi, n, list(range(n))).splitlines()
     assert _compare_text(_state_lines(out.stdout), want) >= 0.95
-    m = re.search(r"CNOT CCX amp\[3\] = (\S+) (\S+)", out.stderr)
-    assert m and abs(complex(float(m.group(1)), float(m.group(2))) - psi[3]) <= 1e-12
+    m = re.search(r"CN CCX amp\[3\] = ([^\s,]+) ([^\s,]+)", out.stderr)
+    assert m, out.stderr[-500:]
+    assert abs(complex(float(m.group(1)), float(m.group(2))) - psi[3]) <= 1e-12

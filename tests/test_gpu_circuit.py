@@ -342,3 +342,35 @@ def test_known_answers_at_full_size(gpu):
         want = complex(math.cos(ph), math.sin(ph)) / math.sqrt(N)
         assert abs(st.get_amp(phys) - want) < 1e-12 / math.sqrt(N) * 100
     c.close()
+
+
+def test_circuits_share_a_resident_state(gpu):
+    """hq_circuit_use_state: circuits run one after another on one
+    caller-owned state (with a work buffer: mixed schedules), which outlives
+    run(destroy=1) and close(); a state of the wrong size is refused."""
+    n = 20
+    st = gpu.State(n, gpu.HQ_C128, with_buffer=True)
+    cost = None
+    for k, spec in enumerate((random_layers(n, 6, 41), blocks_and_layers(n))):
+        c = _circuit(gpu, n, spec)
+        c.set_option("mode", gpu.HQ_MODE_MIX)
+        if k == 1:
+            import tempfile
+            cost = tempfile.NamedTemporaryFile("w", suffix=".out", delete=False)
+            cost.write(flat_cost_tables(0.03))
+            cost.close()
+            c.load_cost_model(cost.name)
+        c.use_state(st)
+        c.run(copy_back=False, destroy=True)
+        assert c.device_state().value == st.handle.value
+        want = oracle.run_plain(to_oracle(spec), n)
+        assert rel_err(c.logical_state(), want) <= 1e-10
+        c.close()
+        assert abs(st.norm2() - 1) < 1e-12   # still allocated after the circuit is gone
+    small = gpu.State(n - 1)
+    c = _circuit(gpu, n, random_layers(n, 1, 1))
+    with pytest.raises(gpu.HQError):
+        c.use_state(small)
+    c.close()
+    small.close()
+    st.close()
