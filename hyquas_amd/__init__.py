@@ -95,6 +95,7 @@ def _declare(L):
         "hq_measure_zero": (S, [c_void_p, c_int, POINTER(c_double)]),
         "hq_state_norm": (S, [c_void_p, POINTER(c_double)]),
         "hq_expect_z": (S, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+        "hq_expect_z_noalloc": (S, [c_void_p, POINTER(c_double), POINTER(c_double)]),
         "hq_state_has_buffer": (S, [c_void_p, POINTER(c_int)]),
         "hq_circuit_use_state": (S, [c_void_p, c_void_p]),
         "hq_state_inner": (S, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_double)]),

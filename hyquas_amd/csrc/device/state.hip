@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace hq {
 
@@ -106,6 +107,10 @@ __global__ void k_expect_z(const cplx<R>* __restrict__ a, int n, double* __restr
             if (threadIdx.x == 0) out[blockIdx.x * stride_out + 1 + q] = r1;
             double r2 = block_sum(s2[q], sh);
             if (threadIdx.x == 0) out[blockIdx.x * stride_out + 1 + RED_MAXQ + q] = r2;
+        } else if (threadIdx.x == 0) {
+            // columns of absent qubits hold zeros: every column is defined
+            out[blockIdx.x * stride_out + 1 + q] = 0.0;
+            out[blockIdx.x * stride_out + 1 + RED_MAXQ + q] = 0.0;
         }
     }
 }
@@ -128,6 +133,64 @@ __global__ void k_inner(const cplx<RA>* __restrict__ a, const cplx<RB>* __restri
     if (threadIdx.x == 0) out[2 * blockIdx.x] = r1;
     double r2 = block_sum(si, sh);
     if (threadIdx.x == 0) out[2 * blockIdx.x + 1] = r2;
+}
+
+// Fixed-order Kahan sum of column c of `count` partial rows (row stride
+// `stride`) into fin[c], one thread per column: the same operations in the
+// same order as the host-side sum of the partials (sum_partials), so both
+// give the same FP64 result bit for bit.
+__global__ void k_sum_columns(const double* __restrict__ part, int count, int stride, int cols,
+                              double* __restrict__ fin)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    double sum = 0, comp = 0;
+    for (int b = 0; b < count; b++) {
+        double y = part[(size_t)b * stride + c] - comp;
+        double t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    fin[c] = sum;
+}
+
+// ---- allocation-free readout -----------------------------------------------------------
+// Scratch of hq_expect_z_noalloc: the per-block partials of k_expect_z (at
+// most RED_Z_BLOCKS rows of 1 + 2 * RED_MAXQ doubles) followed by one row of
+// column sums.  One copy per device, held until the process ends; host code
+// addresses it only through the pointer hipGetSymbolAddress returns for the
+// device (noalloc_scratch below).
+constexpr int RED_Z_BLOCKS = 256;
+constexpr int RED_Z_STRIDE = 1 + 2 * RED_MAXQ;
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)(RED_Z_BLOCKS + 1) * RED_Z_STRIDE;   // 147 KiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with that
+// device current (each device holds its own copy at its own address).
+static hq_status noalloc_scratch(int device, double** out)
+{
+    HQ_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, HQ_ERR_UNSUPPORTED,
+             "hq_expect_z_noalloc: device index beyond the supported 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        HQ_HIP(hipGetDevice(&cur));
+        if (cur != device) HQ_HIP(hipSetDevice(device));
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("hq_expect_z_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return HQ_ERR_HIP;
+        }
+        g_scratch_ptr[device] = (double*)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return HQ_OK;
 }
 
 static hq_status ensure_red(hq_state s, size_t doubles)
@@ -363,33 +426,70 @@ hq_status hq_measure_zero(hq_state s, int target, double* p0)
     return run_norm(s, target, p0);
 }
 
+static void launch_expect_z(hq_state s, unsigned g, double* partials)
+{
+    if (s->precision == HQ_C128)
+        hipLaunchKernelGGL(k_expect_z<double>, dim3(g), dim3(RED_THREADS), 0, s->stream,
+                           (const cplx<double>*)s->data, s->n, partials);
+    else
+        hipLaunchKernelGGL(k_expect_z<float>, dim3(g), dim3(RED_THREADS), 0, s->stream,
+                           (const cplx<float>*)s->data, s->n, partials);
+}
+
+// z[q] = <Z_q>, zz[q] = <Z_q Z_{q+1}> from the column sums of k_expect_z
+// (tot, sum p bit_q, sum p (bit_q xor bit_{q+1})).
+static void finish_expect_z(int n, const double* col, double* z, double* zz)
+{
+    const double tot = col[0];
+    for (int q = 0; q < n; q++) {
+        if (z) z[q] = tot - 2 * col[1 + q];
+        if (zz && q + 1 < n) zz[q] = tot - 2 * col[1 + RED_MAXQ + q];
+    }
+}
+
 hq_status hq_expect_z(hq_state s, double* z, double* zz)
 {
     HQ_CHECK(s, HQ_ERR_ARG, "hq_expect_z: null state");
     HQ_CHECK(s->n <= RED_MAXQ, HQ_ERR_UNSUPPORTED, "hq_expect_z: at most 36 qubits");
-    int stride = 1 + 2 * RED_MAXQ;
-    unsigned g = grid_for(s->amps, RED_THREADS, 256);
+    const int stride = RED_Z_STRIDE;
+    unsigned g = grid_for(s->amps, RED_THREADS, RED_Z_BLOCKS);
     hq_status st = ensure_red(s, (size_t)g * stride);
     if (st != HQ_OK) return st;
-    if (s->precision == HQ_C128)
-        hipLaunchKernelGGL(k_expect_z<double>, dim3(g), dim3(RED_THREADS), 0, s->stream,
-                           (const cplx<double>*)s->data, s->n, s->d_red);
-    else
-        hipLaunchKernelGGL(k_expect_z<float>, dim3(g), dim3(RED_THREADS), 0, s->stream,
-                           (const cplx<float>*)s->data, s->n, s->d_red);
+    launch_expect_z(s, g, s->d_red);
     HQ_HIP(hipGetLastError());
     HQ_HIP(hipMemcpyAsync(s->h_red, s->d_red, (size_t)g * stride * sizeof(double), hipMemcpyDeviceToHost,
                           s->stream));
     HQ_HIP(hipStreamSynchronize(s->stream));
-    double tot;
-    sum_partials(s, (int)g, stride, 0, &tot);
-    for (int q = 0; q < s->n; q++) {
-        double s1, s2;
-        sum_partials(s, (int)g, stride, 1 + q, &s1);
-        sum_partials(s, (int)g, stride, 1 + RED_MAXQ + q, &s2);
-        if (z) z[q] = tot - 2 * s1;
-        if (zz && q + 1 < s->n) zz[q] = tot - 2 * s2;
-    }
+    double col[RED_Z_STRIDE];
+    for (int c = 0; c < stride; c++) sum_partials(s, (int)g, stride, c, &col[c]);
+    finish_expect_z(s->n, col, z, zz);
+    return HQ_OK;
+}
+
+hq_status hq_expect_z_noalloc(hq_state s, double* z, double* zz)
+{
+    HQ_CHECK(s, HQ_ERR_ARG, "hq_expect_z_noalloc: null state");
+    HQ_CHECK(s->n <= RED_MAXQ, HQ_ERR_UNSUPPORTED, "hq_expect_z_noalloc: at most 36 qubits");
+    const int stride = RED_Z_STRIDE;
+    unsigned g = grid_for(s->amps, RED_THREADS, RED_Z_BLOCKS);
+    HQ_CHECK((size_t)(g + 1) * stride <= NOALLOC_SCRATCH_DOUBLES, HQ_ERR_ARG,
+             "hq_expect_z_noalloc: partials exceed the " + std::to_string(NOALLOC_SCRATCH_DOUBLES) +
+                 "-double scratch");
+    double* ptr = nullptr;
+    hq_status st = noalloc_scratch(s->device, &ptr);
+    if (st != HQ_OK) return st;
+    // every partial row and the column-sum row are written before they are
+    // read: nothing in the scratch has to start at zero
+    launch_expect_z(s, g, ptr);
+    HQ_HIP(hipGetLastError());
+    double* fin = ptr + (size_t)g * stride;
+    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(128), 0, s->stream, (const double*)ptr, (int)g, stride,
+                       stride, fin);
+    HQ_HIP(hipGetLastError());
+    double col[RED_Z_STRIDE];
+    HQ_HIP(hipMemcpyAsync(col, fin, stride * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HQ_HIP(hipStreamSynchronize(s->stream));
+    finish_expect_z(s->n, col, z, zz);
     return HQ_OK;
 }
 

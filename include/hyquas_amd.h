@@ -206,6 +206,19 @@ hq_status hq_state_norm(hq_state s, double *norm2);
  * pointer may be NULL.  One HBM pass, FP64 accumulation.  Synchronous. */
 hq_status hq_expect_z(hq_state s, double *z, double *zz);
 
+/* hq_expect_z without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The per-block partials and their
+ * column sums live in a fixed device array of the library (257 rows of 73
+ * doubles, 147 KiB, one copy per device, held until the process ends) instead
+ * of the state's lazily allocated reduction buffers; the caller passes the
+ * state and the output arrays as for hq_expect_z.  Same values as
+ * hq_expect_z bit for bit (same partials, same summation order).  Every call
+ * on one device shares that device's array: calls that overlap on one device
+ * (several host threads, or states on different streams) race.  States of at
+ * most 36 qubits.  Synchronous. */
+hq_status hq_expect_z_noalloc(hq_state s, double *z, double *zz);
+
 /* Overlap <a|b> of two states with the same qubit count (any precisions),
  * FP64 accumulation.  Synchronous. */
 hq_status hq_state_inner(hq_state a, hq_state b, double *re, double *im);

@@ -29,3 +29,11 @@ def test_python_binding_covers_header(hq):
 def test_struct_layout(hq):
     assert ctypes.sizeof(hq.HQGate) == 80
     assert hq.HQGate.mat.offset == 16
+
+
+def test_noalloc_entry_point_rejects_null_state(hq):
+    # argument checks come before any HIP call: no GPU needed
+    z = (ctypes.c_double * 4)()
+    rc = hq.lib().hq_expect_z_noalloc(None, z, None)
+    assert rc == hq.HQ_ERR_ARG
+    assert b"null state" in hq.lib().hq_last_error()
