@@ -18,6 +18,15 @@ events recorded on the state's stream around every launch in the timed steps).
 cpu_baseline: the FP64 OpenMP per-gate CPU path (oracle/, kernelSimple.cu
 semantics) timed on the first --cpu-gates gates of the same 30-qubit
 circuit on this host, scaled to the whole circuit.
+
+The default line also carries the metric's second half (--no-forms skips
+it): "batch", 4096 hardware-efficient-ansatz circuits batched in complex64
+against the FP64 hybrid path one circuit at a time on resident states and
+the plain per-gate path (both on 128 evenly spaced circuits of the batch);
+"serve", a Poisson stream of 500 requests at each of --form-rates served by
+the complex64 engine against the FP64 hybrid path one circuit at a time
+(circuits/s, p50, p99 latency).  --form batch|serve|precision run one form
+alone with more detail.
 """
 from __future__ import annotations
 
@@ -109,15 +118,18 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256,
                     help="batch: circuits of the plain FP64 one-at-a-time baseline (evenly spaced)")
     ap.add_argument("--no-autotune", action="store_true", help="batch: default tiles instead of autotuned ones")
+    ap.add_argument("--batch-jit", type=int, choices=[0, 1], default=0,
+                    help="batch: 1 specialised batched kernels shared by same-structure groups, "
+                         "0 the interpreter kernel with autotuned tiles (default)")
     ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
     ap.add_argument("--rates", default="20,100", help="serve: arrival rates (requests/s), comma separated")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain per-gate baseline")
     ap.add_argument("--no-forms", action="store_true",
                     help="default form: skip the batching and serving summaries")
-    ap.add_argument("--form-count", type=int, default=1024,
+    ap.add_argument("--form-count", type=int, default=4096,
                     help="default form: circuits of the batching summary")
-    ap.add_argument("--form-rate", type=float, default=100.0,
-                    help="default form: arrival rate (requests/s) of the serving summary")
+    ap.add_argument("--form-rates", default="100,400",
+                    help="default form: arrival rates (requests/s) of the serving summary")
     ap.add_argument("--copy-pass", action="store_true",
                     help="before timing, run one zero-gate OShareMem pass (a copy of known size, "
                          "2 * 2^n * amplitude bytes) to calibrate HBM counters")
@@ -250,9 +262,11 @@ def main():
         # the metric's second half: the batched and served complex64 forms
         # against FP64 one circuit at a time on the same GPU
         c.close()
+        rates = [float(x) for x in a.form_rates.split(",") if x]
         forms = {"batch": batch_form(a, hq, a.form_count, 128, precisions=("c64",))[0],
-                 "serve": serve_form(a, hq, a.form_rate, a.requests, a.seed, plain=False)}
-        forms["serve"]["workload"] = SERVE_WORKLOAD.format(requests=a.requests, seed=a.seed)
+                 "serve": {"workload": SERVE_WORKLOAD.format(requests=a.requests, seed=a.seed),
+                           "sweep": [serve_form(a, hq, r, a.requests, a.seed + k, plain=False)
+                                     for k, r in enumerate(rates)]}}
 
     out = {
         "metric": "30q fp64 sim time + % HBM roofline; served circuits/s + p99 vs plain",
@@ -307,6 +321,32 @@ def _fidelity(hq, spec, kinds="hea"):
     return 1.0 - abs(ov) ** 2
 
 
+def _build_batch(hq, e, circuits, name):
+    """e.batch(circuits) on a worker thread; while it compiles, report how many
+    specialised kernels the engine has built (the compile is the long part of
+    a cold batch)."""
+    import threading
+    box = {}
+
+    def work():
+        try:
+            box["b"] = e.batch(circuits, hq.HQ_OUT_EXPECT)
+        except Exception as ex:  # re-raised below
+            box["err"] = ex
+
+    th = threading.Thread(target=work)
+    t0 = time.perf_counter()
+    th.start()
+    while th.is_alive():
+        th.join(10.0)
+        if th.is_alive():
+            _progress(f"batch: {name} building, {e.kernels()} specialised kernels compiled "
+                      f"({time.perf_counter() - t0:.0f} s)")
+    if "err" in box:
+        raise box["err"]
+    return box["b"]
+
+
 def _one_at_a_time(hq, specs, kind, mode):
     """Circuits of `specs` one at a time on the same GPU, complex128: `mode`
     "hybrid" = the hybrid partitioner (interpreter kernels, no per-circuit
@@ -355,10 +395,17 @@ def batch_form(a, hq, count, plain_sample, precisions=("c64", "c128")):
         prec = hq.HQ_C64 if name == "c64" else hq.HQ_C128
         e = hq.Engine(prec, 280 * 10 ** 9)
         t0 = time.perf_counter()
-        tiles = {} if a.no_autotune else autotune_tiles(hq, e, specs, circuits)
+        if a.batch_jit:
+            # specialised batched kernels (one per group structure) on the
+            # default tiles: the whole state up to 13 qubits, else 12
+            e.set_option("compile_threads", 16)
+            e.set_option("jit", 1)
+            tiles = {}
+        else:
+            tiles = {} if a.no_autotune else autotune_tiles(hq, e, specs, circuits)
         tune_s = time.perf_counter() - t0
         t0 = time.perf_counter()
-        b = e.batch(circuits, hq.HQ_OUT_EXPECT)
+        b = _build_batch(hq, e, circuits, name)
         build_s = time.perf_counter() - t0
         for _ in range(max(1, a.warmup)):
             b.run()
@@ -369,7 +416,8 @@ def batch_form(a, hq, count, plain_sample, precisions=("c64", "c128")):
         ex, _ = b.results()
         zz[name] = ex[:, 2].copy()
         info = b.info()
-        res[name] = {"circuits_per_s": round(count / wall, 1), "s_per_batch": round(wall, 6),
+        res[name] = {"kernels": "specialised" if a.batch_jit else "interpreter",
+                     "circuits_per_s": round(count / wall, 1), "s_per_batch": round(wall, 6),
                      "device_ms": round(min(dev), 3), "rounds": info["rounds"], "launches": info["launches"],
                      "device_bytes": info["device_bytes"], "build_s": round(build_s, 3),
                      "autotune_s": round(tune_s, 3), "tiles": tiles,

@@ -98,6 +98,7 @@ def _declare(L):
         "hq_expect_z_noalloc": (S, [c_void_p, POINTER(c_double), POINTER(c_double)]),
         "hq_state_has_buffer": (S, [c_void_p, POINTER(c_int)]),
         "hq_circuit_use_state": (S, [c_void_p, c_void_p]),
+        "hq_engine_kernels": (S, [c_void_p, POINTER(c_int64)]),
         "hq_state_inner": (S, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_double)]),
         "hq_profile_enable": (S, [c_void_p, c_int]),
         "hq_profile_read": (S, [c_void_p, POINTER(c_int), POINTER(c_double), c_int, POINTER(c_int)]),
@@ -740,6 +741,12 @@ class Engine:
         t = c_double()
         _check(lib().hq_engine_now(self.handle, byref(t)))
         return t.value
+
+    def kernels(self) -> int:
+        """Specialised batched kernels compiled so far (option "jit")."""
+        k = c_int64()
+        _check(lib().hq_engine_kernels(self.handle, byref(k)))
+        return k.value
 
     def batch(self, circuits, outputs: int = HQ_OUT_EXPECT) -> "Batch":
         return Batch(self, circuits, outputs)

@@ -348,13 +348,72 @@ struct EngineOptions {
     int coalesce = 4;
     int compile_threads = 4;
     int max_inflight = 2;     // rounds queued on the device ahead of the host
+    int jit = 0;              // 1: specialised batched kernels, shared by same-structure groups
     int tile_by_n[65] = {0};  // autotuned tile per qubit count (0: the rule above)
 };
 
 struct GroupPlan {
     int rb = 0, tb = 0, k = 0;
     uint32_t nops = 0;
-    uint64_t blob_off = 0;
+    uint64_t blob_off = 0;        // the interpreter's program, or the specialised kernel's coefficients
+    hipFunction_t fn = nullptr;   // specialised batched kernel (null: the interpreter runs the group)
+};
+
+// Specialised batched kernels of an engine, one per distinct source: groups
+// of the same structure (gate types, qubits, layout) emit the same source and
+// share the kernel; their angles stay in each job's coefficient array.
+struct KernelCache {
+    struct Entry {
+        hipModule_t mod = nullptr;
+        hipFunction_t fn = nullptr;
+        bool ready = false;
+        hq_status st = HQ_OK;
+        std::string err;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::map<std::string, std::unique_ptr<Entry>> map;
+    int device = 0;
+    size_t compiled = 0;
+
+    ~KernelCache()
+    {
+        for (auto& kv : map)
+            if (kv.second->mod) (void)hipModuleUnload(kv.second->mod);
+    }
+    // the kernel of `src`: compiled and loaded by the first caller, waited
+    // for by concurrent ones
+    hq_status get(const std::string& src, hipFunction_t* fn)
+    {
+        Entry* e;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            auto it = map.find(src);
+            if (it != map.end()) {
+                e = it->second.get();
+                cv.wait(lk, [e] { return e->ready; });
+                if (e->st != HQ_OK) { set_error(e->err); return e->st; }
+                *fn = e->fn;
+                return HQ_OK;
+            }
+            e = (map[src] = std::make_unique<Entry>()).get();
+        }
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+        hipModule_t mod = nullptr;
+        hipFunction_t f = nullptr;
+        hq_status st = jit_compile_load(src, "hq_group_batch", &mod, &f);
+        std::lock_guard<std::mutex> lk(mu);
+        e->mod = mod;
+        e->fn = f;
+        e->st = st;
+        if (st != HQ_OK) e->err = get_error();
+        else compiled++;
+        e->ready = true;
+        cv.notify_all();
+        *fn = f;
+        return st;
+    }
 };
 
 struct JobPlan {
@@ -391,22 +450,72 @@ static void append_blob(JobPlan* jp, const ShmPlan& plan)
     jp->groups.push_back(g);
 }
 
+// A group run by a specialised batched kernel: its coefficients (state precision)
+// take the program's place in the job's blob area.
+static void append_jit(JobPlan* jp, const ShmPlan& plan, const std::vector<double>& coef, hipFunction_t fn)
+{
+    GroupPlan g;
+    g.rb = plan.rb;
+    g.tb = plan.tb;
+    g.k = plan.k;
+    g.nops = (uint32_t)plan.ops.size();
+    g.fn = fn;
+    const size_t rbytes = plan.precision == HQ_C128 ? 8 : 4;
+    size_t off = (jp->blobs.size() + 255) & ~(size_t)255;
+    jp->blobs.resize(off + std::max<size_t>(coef.size(), 1) * rbytes, 0);
+    char* d = jp->blobs.data() + off;
+    for (size_t i = 0; i < coef.size(); i++) {
+        if (rbytes == 8) memcpy(d + 8 * i, &coef[i], 8);
+        else { const float f = (float)coef[i]; memcpy(d + 4 * i, &f, 4); }
+    }
+    g.blob_off = off;
+    jp->groups.push_back(g);
+}
+
+// One group's program: the specialised batched kernel when `kc` is given and
+// it compiles, else the interpreter's op table.
+static hq_status add_group(JobPlan* jp, int n, int precision, const std::vector<hq_gate>& hg, uint64_t mask,
+                           bool first, bool last, uint64_t zero_mask, KernelCache* kc)
+{
+    ShmPlan plan;
+    hq_status st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), mask, 0, &plan, kc && first, 0, kc != nullptr);
+    if (st != HQ_OK) return st;
+    plan.hdr.zero_mask = zero_mask;
+    // groups of more than kJitMaxOps ops stay with the interpreter (compile time)
+    constexpr size_t kJitMaxOps = 512;
+    if (kc && plan.ops.size() <= kJitMaxOps) {
+        plan.init = first;
+        std::vector<double> coef;
+        const std::string src = jit_source(plan, &coef, true, last);
+        hipFunction_t fn = nullptr;
+        if (!src.empty() && kc->get(src, &fn) == HQ_OK && fn) {
+            append_jit(jp, plan, coef, fn);
+            return HQ_OK;
+        }
+        // the interpreter takes the group when its kernel cannot be built
+    }
+    append_blob(jp, plan);
+    return HQ_OK;
+}
+
 static hq_status compile_job(const std::vector<Gate>& gates, int n, int precision, const EngineOptions& o,
-                             JobPlan* jp)
+                             JobPlan* jp, KernelCache* kc = nullptr)
 {
     HQ_CHECK(n >= 7 && n <= 34, HQ_ERR_ARG, "engine: circuits must have 7..34 qubits");
     jp->n = n;
     const int k = tile_for(o, n);
     HQ_CHECK(k >= 7 && k <= 13, HQ_ERR_ARG, "engine: tile must have 7..13 qubits");
+    // specialised kernels for partitioned circuits only: a whole-state group
+    // holds every gate of its circuit (straight-line code of thousands of
+    // gates compiles slowly and is rarely shared); such circuits are small
+    // and stay with the interpreter
+    if (!o.jit || k == n) kc = nullptr;
     if (k == n) {
         std::vector<hq_gate> hg;
         hg.reserve(gates.size());
         for (const Gate& g : gates) hg.push_back(g.toHQ());
-        ShmPlan plan;
-        hq_status st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), (n >= 64 ? ~0ull : (1ull << n) - 1), 0,
-                                     &plan);
+        hq_status st = add_group(jp, n, precision, hg, (n >= 64 ? ~0ull : (1ull << n) - 1), true, true, 0, kc);
         if (st != HQ_OK) return st;
-        append_blob(jp, plan);
         jp->k_last = k;
         return HQ_OK;
     }
@@ -422,7 +531,8 @@ static hq_status compile_job(const std::vector<Gate>& gates, int n, int precisio
     // passes skip them
     const uint64_t all = (1ull << n) - 1;
     uint64_t support = 0;
-    bool first = true;
+    size_t total = 0, idx = 0;
+    for (auto& lg : sch.localGroups) total += lg.fullGroups.size();
     for (auto& lg : sch.localGroups) {
         for (auto& gg : lg.fullGroups) {
             HQ_CHECK(gg.backend == Backend::PerGate, HQ_ERR_ARG, "engine: unexpected TransMM group");
@@ -430,22 +540,18 @@ static hq_status compile_job(const std::vector<Gate>& gates, int n, int precisio
             std::vector<hq_gate> hg;
             hg.reserve(gg.gates.size());
             for (const Gate& g : gg.gates) hg.push_back(g.toHQ());
-            ShmPlan plan;
-            hq_status st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), (uint64_t)rel, 0, &plan);
+            const bool first = idx == 0, last = idx + 1 == total;
+            hq_status st = add_group(jp, n, precision, hg, (uint64_t)rel, first, last, first ? 0 : all & ~support, kc);
             if (st != HQ_OK) return st;
-            if (!first) plan.hdr.zero_mask = all & ~support;
             for (const Gate& g : gg.gates)
                 if (!g.isDiagonal()) support |= 1ull << g.targetQubit;
-            first = false;
-            append_blob(jp, plan);
+            idx++;
         }
     }
     if (jp->groups.empty()) {
         // no gates: one empty pass still produces the readout
-        ShmPlan plan;
-        hq_status st = plan_sharemem(n, precision, nullptr, 0, (1ull << k) - 1, 0, &plan);
+        hq_status st = add_group(jp, n, precision, std::vector<hq_gate>(), (1ull << k) - 1, true, true, 0, kc);
         if (st != HQ_OK) return st;
-        append_blob(jp, plan);
     }
     jp->k_last = jp->groups.back().k;
     return HQ_OK;
@@ -491,11 +597,12 @@ static std::vector<uint64_t> topk_levels(uint64_t elems)
 }
 
 struct LaunchDesc {
-    int kind;         // 0 group pass, 1 finish, 2 top-k
+    int kind;         // 0 group pass (interpreter), 1 finish, 2 top-k, 3 group pass (specialised kernel)
     int rb, tb;
     uint64_t table_off;
     uint32_t count;
     uint64_t blocks;
+    hipFunction_t fn = nullptr;
 };
 
 struct RoundBuild {
@@ -521,6 +628,7 @@ static void build_round(const std::vector<Job*>& jobs, uint64_t res_dev, RoundBu
         int rb, tb;
         uint64_t tiles;
         uint32_t cost;
+        hipFunction_t fn;
     };
     std::vector<Entry> ents;
     for (Job* j : jobs) {
@@ -538,11 +646,13 @@ static void build_round(const std::vector<Job*>& jobs, uint64_t res_dev, RoundBu
         e.tb = gp.tb;
         e.tiles = 1ull << (j->n - gp.k);
         e.cost = gp.nops;
+        e.fn = gp.fn;
         ents.push_back(e);
         if (last) rb->finishing.push_back(j);
     }
-    // one launch per shape; long blocks first
+    // one launch per interpreter shape or specialised kernel; long blocks first
     std::stable_sort(ents.begin(), ents.end(), [](const Entry& a, const Entry& b) {
+        if (a.fn != b.fn) return std::less<const void*>()(a.fn, b.fn);
         if (a.rb != b.rb) return a.rb < b.rb;
         if (a.tb != b.tb) return a.tb < b.tb;
         return a.cost > b.cost;
@@ -551,7 +661,7 @@ static void build_round(const std::vector<Job*>& jobs, uint64_t res_dev, RoundBu
         size_t e = i;
         uint64_t blocks = 0;
         std::vector<BatchItem> items;
-        while (e < ents.size() && ents[e].rb == ents[i].rb && ents[e].tb == ents[i].tb) {
+        while (e < ents.size() && ents[e].fn == ents[i].fn && ents[e].rb == ents[i].rb && ents[e].tb == ents[i].tb) {
             BatchItem it = ents[e].it;
             it.first_block = blocks;
             blocks += ents[e].tiles;
@@ -559,7 +669,8 @@ static void build_round(const std::vector<Job*>& jobs, uint64_t res_dev, RoundBu
             e++;
         }
         LaunchDesc d;
-        d.kind = 0;
+        d.kind = ents[i].fn ? 3 : 0;
+        d.fn = ents[i].fn;
         d.rb = ents[i].rb;
         d.tb = ents[i].tb;
         d.table_off = table_append(rb->table, items.data(), items.size() * sizeof(BatchItem));
@@ -617,6 +728,11 @@ static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int preci
             rc = precision == HQ_C128
                      ? launch_batch_shape<double>(d.rb, d.tb, (const BatchItem*)tab, d.count, d.blocks, st)
                      : launch_batch_shape<float>(d.rb, d.tb, (const BatchItem*)tab, d.count, d.blocks, st);
+        } else if (d.kind == 3) {
+            const void* items = tab;
+            uint32_t count = d.count;
+            void* args[] = {&items, &count};
+            HQ_HIP(hipModuleLaunchKernel(d.fn, (unsigned)d.blocks, 1, 1, 1u << d.tb, 1, 1, 0, st, args, nullptr));
         } else if (d.kind == 1) {
             hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)d.blocks), dim3(256), 0, st, (const FinishItem*)tab);
             HQ_HIP(hipGetLastError());
@@ -726,6 +842,7 @@ using namespace hq;
 struct hq_engine_s {
     int precision = HQ_C128;
     EngineOptions opt;
+    KernelCache kernels;               // specialised batched kernels (option "jit")
     Arena arena;
     hipStream_t stream = nullptr;
     double epoch = 0;
@@ -776,7 +893,7 @@ struct hq_engine_s {
             hq_status st;
             std::string err;
             try {
-                st = compile_job(j->gates, j->n, precision, o, &jp);
+                st = compile_job(j->gates, j->n, precision, o, &jp, &kernels);
                 if (st != HQ_OK) err = get_error();
             } catch (const std::exception& e) {
                 st = HQ_ERR_ARG;
@@ -1001,6 +1118,7 @@ hq_status hq_engine_create(int precision, uint64_t hbm_budget, hq_engine* out)
     hq_status st = hq_device_count(&ndev);
     if (st != HQ_OK) return st;
     std::unique_ptr<hq_engine_s> e(new hq_engine_s());
+    if (hipGetDevice(&e->kernels.device) != hipSuccess) e->kernels.device = 0;
     e->precision = precision;
     uint64_t chunk = 1ull << 33;  // 8 GiB
     while (chunk > hbm_budget) chunk >>= 1;
@@ -1037,6 +1155,9 @@ hq_status hq_engine_set_option(hq_engine e, const char* key, int64_t value)
         HQ_CHECK(value >= 1 && value <= 16, HQ_ERR_ARG, "compile_threads must be in [1, 16]");
         HQ_CHECK(e->workers.empty(), HQ_ERR_ARG, "compile_threads must be set before the first submit");
         e->opt.compile_threads = (int)value;
+    } else if (k == "jit") {
+        HQ_CHECK(value == 0 || value == 1, HQ_ERR_ARG, "jit must be 0 or 1");
+        e->opt.jit = (int)value;
     } else if (k == "max_inflight") {
         HQ_CHECK(value >= 1 && value <= 8, HQ_ERR_ARG, "max_inflight must be in [1, 8]");
         e->opt.max_inflight = (int)value;
@@ -1044,6 +1165,14 @@ hq_status hq_engine_set_option(hq_engine e, const char* key, int64_t value)
         set_error("hq_engine_set_option: unknown option " + k);
         return HQ_ERR_ARG;
     }
+    return HQ_OK;
+}
+
+hq_status hq_engine_kernels(hq_engine e, int64_t* compiled)
+{
+    HQ_CHECK(e && compiled, HQ_ERR_ARG, "hq_engine_kernels: null argument");
+    std::lock_guard<std::mutex> lk(e->kernels.mu);
+    *compiled = (int64_t)e->kernels.compiled;
     return HQ_OK;
 }
 
@@ -1234,7 +1363,8 @@ hq_status hq_batch_create(hq_engine e, const hq_circuit* circuits, int count, in
             int i = next++;
             if (i >= count) return;
             try {
-                sts[i] = compile_job(circuits[i]->c->getGates(), b->jobs[i]->n, e->precision, o, &b->jobs[i]->plan);
+                sts[i] = compile_job(circuits[i]->c->getGates(), b->jobs[i]->n, e->precision, o, &b->jobs[i]->plan,
+                                     &e->kernels);
                 if (sts[i] != HQ_OK) errs[i] = get_error();
             } catch (const std::exception& ex) {
                 sts[i] = HQ_ERR_ARG;

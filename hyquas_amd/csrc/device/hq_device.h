@@ -175,12 +175,21 @@ int sharemem_default_rb(int precision, int tile);
 // io: bit 0 adds a leading, bit 1 a trailing stage without gates where the
 // first / last stage holds one of the three lowest tile bits in registers
 // (the HBM access then covers whole 128-byte lines per wave instruction).
+// shared_code: op classes that do not depend on gate angles (rotations keep
+// their full 2x2 form), so same-structure groups share a specialised kernel.
 hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gates, uint64_t shared_mask,
-                        int rb, ShmPlan* out, bool init = false, int io = 0);
+                        int rb, ShmPlan* out, bool init = false, int io = 0, bool shared_code = false);
 // Specialised kernel for a planned group: generate, compile (cached), load.
 hq_status jit_sharemem(const ShmPlan& plan, hq_program p);
 hq_status launch_sharemem_jit(hq_state s, hq_program p);
 hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std::vector<hq_program>& progs,
                             int threads);
-std::string jit_source(const ShmPlan& plan, std::vector<double>* coef);
+// batch: the batched form hq_group_batch(const BatchItem* items, unsigned count)
+// (blocks map to (circuit, tile) by the items' first_block, the item's prog
+// field holds the group's coefficient array); readout: it also folds the
+// tile's sum p, p Z, p ZZ into the item's partials (the circuit's last group).
+std::string jit_source(const ShmPlan& plan, std::vector<double>* coef, bool batch = false, bool readout = false);
+// Compile a generated source (cached like every specialised kernel) and load
+// its kernel `name` as a module of the current device.
+hq_status jit_compile_load(const std::string& src, const char* name, hipModule_t* mod, hipFunction_t* fn);
 }  // namespace hq

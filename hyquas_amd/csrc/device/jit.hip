@@ -76,8 +76,11 @@ struct Emitter {
     std::vector<int> name;   // register j is held in variable <gen><name[j]>
     char gen = 'r';          // variable set of the current registers (r or q)
     bool split;              // tile above 64 KiB: two-phase LDS exchanges over half the tile
+    bool batch;              // batched form: blocks map to (circuit, tile) through an item table
+    bool readout;            // batched form, last group: fold sum p, p Z, p ZZ into per-tile partials
 
-    Emitter(const ShmPlan& p, std::vector<double>* c) : P(p), coef(c)
+    Emitter(const ShmPlan& p, std::vector<double>* c, bool batch_form = false, bool readout_fold = false)
+        : P(p), coef(c), batch(batch_form), readout(readout_fold)
     {
         NR = 1 << p.rb;
         NT = 1 << p.tb;
@@ -89,7 +92,7 @@ struct Emitter {
         nt_ld = nt_st = p.nt;
         const char* bm = getenv("HQ_JIT_BMAP");
         bmap = bm ? atoi(bm) : 0;
-        if ((bmap == 2 && P.hdr.num_block_bits < 3) || (bmap == 1 && P.hdr.num_block_bits == 0)) bmap = 0;
+        if ((bmap == 2 && P.hdr.num_block_bits < 3) || (bmap == 1 && P.hdr.num_block_bits == 0) || batch) bmap = 0;
         name.resize(NR);
         for (int j = 0; j < NR; j++) name[j] = j;
     }
@@ -345,6 +348,9 @@ struct Emitter {
     std::string emit()
     {
         const int nst = (int)P.stages.size();
+        // HQ_G: global address space for pointers the batched form reads from
+        // its item table (generic pointers would make every load per-lane)
+        o << "#ifndef HQ_G\n#define HQ_G" << (batch ? " __attribute__((address_space(1)))" : "") << "\n#endif\n";
         o << "typedef " << (f32 ? "float" : "double") << " R;\n";
         o << "struct __attribute__((aligned(" << (f32 ? 8 : 16) << "))) C { R x, y; };\n";
         const int pp_ld = P.init ? -1 : pair_pos(P.stages[0]);
@@ -352,18 +358,36 @@ struct Emitter {
         if (nt_ld || nt_st) o << "typedef R V2 __attribute__((ext_vector_type(2)));\n";
         if (pp_ld >= 0 || pp_st >= 0) o << "typedef float F4 __attribute__((ext_vector_type(4)));\n";
         if (nt_ld)
-            o << "__device__ __forceinline__ C ld(const C* p) { const V2 v = __builtin_nontemporal_load((const V2*)p); "
+            o << "__device__ __forceinline__ C ld(const HQ_G C* p) { const V2 v = __builtin_nontemporal_load((const HQ_G V2*)p); "
                  "C c; c.x = v.x; c.y = v.y; return c; }\n";
         if (nt_st)
-            o << "__device__ __forceinline__ void st(C* p, const C c) { V2 v; v.x = c.x; v.y = c.y; "
-                 "__builtin_nontemporal_store(v, (V2*)p); }\n";
+            o << "__device__ __forceinline__ void st(HQ_G C* p, const C c) { V2 v; v.x = c.x; v.y = c.y; "
+                 "__builtin_nontemporal_store(v, (HQ_G V2*)p); }\n";
         // two waves per SIMD (two blocks per CU): a kernel that would need more
         // than 256 registers spills a few instead of halving the occupancy
-        o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2)))"
-          << " void hq_group(C* __restrict__ psi, const R* __restrict__ cf)\n{\n";
+        if (batch) {
+            // same layout as the engine's BatchItem; `cf` is this group's
+            // coefficient array, mode bit 1 = no store (the state is not needed)
+            o << "struct BItem { unsigned long long psi, cf, part, first_block; unsigned n, mode; };\n";
+            o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2)))"
+              << " void hq_group_batch(const BItem* __restrict__ items, unsigned count)\n{\n";
+        } else {
+            o << "extern \"C\" __global__ __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(2)))"
+              << " void hq_group(HQ_G C* __restrict__ psi, const HQ_G R* __restrict__ cf)\n{\n";
+        }
         if (nst > 1) o << "    __shared__ C lds[" << (1 << (split ? K - 1 : K)) << "];\n";
         o << "    const unsigned t = threadIdx.x;\n";
-        if (bmap == 2)
+        if (batch) {
+            o << "    unsigned lo_ = 0, hi_ = count;\n";
+            o << "    const unsigned long long bid = blockIdx.x;\n";
+            o << "    while (hi_ - lo_ > 1) { const unsigned mid = (lo_ + hi_) >> 1; "
+                 "if (items[mid].first_block <= bid) lo_ = mid; else hi_ = mid; }\n";
+            o << "    HQ_G C* __restrict__ psi = (HQ_G C*)items[lo_].psi;\n";
+            o << "    const HQ_G R* __restrict__ cf = (const HQ_G R*)items[lo_].cf;\n";
+            if (readout) o << "    HQ_G double* __restrict__ part = (HQ_G double*)items[lo_].part;\n";
+            o << "    const unsigned mode = items[lo_].mode;\n";
+            o << "    const unsigned long long b = bid - items[lo_].first_block;\n";
+        } else if (bmap == 2)
             o << "    const unsigned long long b = (unsigned long long)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);\n";
         else if (bmap == 1)
             o << "    const unsigned long long b = __builtin_bitreverse64((unsigned long long)blockIdx.x) >> "
@@ -378,30 +402,36 @@ struct Emitter {
             o << "    base |= ((b >> " << j << ") & " << hex64((1ull << len) - 1) << ") << " << p << ";\n";
             j += len - 1;
         }
+        // batched form, readout group: a tile that is zero contributes zero
+        // partials (the per-circuit sum runs over every tile)
+        const std::string zero_part = readout ? "    if (t == 0) { part[3 * b] = 0.0; part[3 * b + 1] = 0.0; part[3 * b + 2] = 0.0; }\n"
+                                              : std::string();
+        // batched form: mode bit 1 = the state is not needed afterwards (no store)
+        const std::string store_if = batch ? "    if (!(mode & 2u)) {\n" : "    {\n";
         if (P.hdr.zero_mask) {
             // tiles known to be zero stay zero: nothing to load or store
-            o << "    if (base & " << hex64(P.hdr.zero_mask) << ") return;\n";
+            o << "    if (base & " << hex64(P.hdr.zero_mask) << ") {\n" << zero_part << "    return;\n    }\n";
         }
         if (P.init && P.hdr.num_block_bits > 0) {
             // first group of a run: every tile but the one holding |0...0>
             // starts and (the group's gates being linear) stays zero
             const ShmStage& st = P.stages[nst - 1];
-            o << "    if (base != 0) {\n";
+            o << "    if (base != 0) {\n" << zero_part << store_if;
             lane_phys(st, "lz");
-            o << "    C* __restrict__ dz = psi + (base + lz);\n";
+            o << "    HQ_G C* __restrict__ dz = psi + (base + lz);\n";
             o << "    C z; z.x = R(0); z.y = R(0);\n";
             for (int j = 0; j < NR; j++) {
                 if (pp_st >= 0) {
                     if ((j >> pp_st) & 1) continue;
                     o << "    { F4 v; v.x = 0.f; v.y = 0.f; v.z = 0.f; v.w = 0.f; ";
-                    if (nt_st) o << "__builtin_nontemporal_store(v, (F4*)(dz + " << reg_off(st, j) << "ull)); }\n";
-                    else o << "*(F4*)(dz + " << reg_off(st, j) << "ull) = v; }\n";
+                    if (nt_st) o << "__builtin_nontemporal_store(v, (HQ_G F4*)(dz + " << reg_off(st, j) << "ull)); }\n";
+                    else o << "*(HQ_G F4*)(dz + " << reg_off(st, j) << "ull) = v; }\n";
                     continue;
                 }
                 if (nt_st) o << "    st(dz + " << reg_off(st, j) << "ull, z);\n";
                 else o << "    dz[" << reg_off(st, j) << "ull] = z;\n";
             }
-            o << "    return;\n    }\n";
+            o << "    }\n    return;\n    }\n";
         }
         o << "    C";
         for (int j = 0; j < NR; j++) o << (j ? ", " : " ") << "r" << j;
@@ -417,14 +447,14 @@ struct Emitter {
                 o << "    r0.x = (base + lp) == 0 ? R(1) : R(0); r0.y = R(0);\n";
                 for (int j = 1; j < NR; j++) o << "    r" << j << ".x = R(0); r" << j << ".y = R(0);\n";
             } else {
-                o << "    const C* __restrict__ src = psi + (base + lp);\n";
+                o << "    const HQ_G C* __restrict__ src = psi + (base + lp);\n";
                 for (int j = 0; j < NR; j++) {
                     if (pp_ld >= 0) {
                         if ((j >> pp_ld) & 1) continue;
                         const int jh = j | (1 << pp_ld);
                         o << "    { const F4 v = ";
-                        if (nt_ld) o << "__builtin_nontemporal_load((const F4*)(src + " << reg_off(st, j) << "ull));";
-                        else o << "*(const F4*)(src + " << reg_off(st, j) << "ull);";
+                        if (nt_ld) o << "__builtin_nontemporal_load((const HQ_G F4*)(src + " << reg_off(st, j) << "ull));";
+                        else o << "*(const HQ_G F4*)(src + " << reg_off(st, j) << "ull);";
                         o << " r" << j << ".x = v.x; r" << j << ".y = v.y; r" << jh << ".x = v.z; r" << jh
                           << ".y = v.w; }\n";
                         continue;
@@ -459,28 +489,59 @@ struct Emitter {
             }
             emit_stage_ops(st);
         }
+        const ShmStage& last = P.stages[nst - 1];
+        if (readout) emit_readout(last);
         {
-            const ShmStage& st = P.stages[nst - 1];
-            o << "    {\n";
-            lane_phys(st, "lq");
-            o << "    C* __restrict__ dst = psi + (base + lq);\n";
+            o << store_if;
+            lane_phys(last, "lq");
+            o << "    HQ_G C* __restrict__ dst = psi + (base + lq);\n";
             for (int j = 0; j < NR; j++) {
                 if (pp_st >= 0) {
                     if ((j >> pp_st) & 1) continue;
                     const int jh = j | (1 << pp_st);
                     o << "    { F4 v; v.x = " << r(j) << ".x; v.y = " << r(j) << ".y; v.z = " << r(jh) << ".x; v.w = "
                       << r(jh) << ".y; ";
-                    if (nt_st) o << "__builtin_nontemporal_store(v, (F4*)(dst + " << reg_off(st, j) << "ull)); }\n";
-                    else o << "*(F4*)(dst + " << reg_off(st, j) << "ull) = v; }\n";
+                    if (nt_st) o << "__builtin_nontemporal_store(v, (HQ_G F4*)(dst + " << reg_off(last, j) << "ull)); }\n";
+                    else o << "*(HQ_G F4*)(dst + " << reg_off(last, j) << "ull) = v; }\n";
                     continue;
                 }
-                if (nt_st) o << "    st(dst + " << reg_off(st, j) << "ull, " << r(j) << ");\n";
-                else o << "    dst[" << reg_off(st, j) << "ull] = " << r(j) << ";\n";
+                if (nt_st) o << "    st(dst + " << reg_off(last, j) << "ull, " << r(j) << ");\n";
+                else o << "    dst[" << reg_off(last, j) << "ull] = " << r(j) << ";\n";
             }
             o << "    }\n";
         }
         o << "}\n";
         return o.str();
+    }
+
+    // Readout of the batched form: per tile, sum p, sum p (number of |0>
+    // qubits minus |1> qubits) and sum p (equal minus unequal neighbour
+    // pairs) over the tile's amplitudes (p = |a|^2, FP64), then a fixed-order
+    // tree over the block's threads; thread 0 writes the tile's three partials.
+    void emit_readout(const ShmStage& st)
+    {
+        const int n = P.n;
+        const uint64_t nn1 = n >= 2 ? ((1ull << (n - 1)) - 1) : 0;
+        o << "    {\n";
+        lane_phys(st, "lr");
+        o << "    double s0 = 0.0, s1 = 0.0, s2 = 0.0;\n";
+        for (int j = 0; j < NR; j++) {
+            o << "    { const unsigned long long ix = base + lr + " << reg_off(st, j) << "ull; const double re = (double)"
+              << r(j) << ".x, im = (double)" << r(j) << ".y; const double p = re * re + im * im; "
+              << "s0 += p; s1 += p * (double)(" << n << " - 2 * (int)__builtin_popcountll(ix)); "
+              << "s2 += p * (double)(" << (n - 1) << " - 2 * (int)__builtin_popcountll((ix ^ (ix >> 1)) & "
+              << hex64(nn1) << ")); }\n";
+        }
+        o << "    __shared__ double red[" << 3 * NT << "];\n";
+        o << "    red[t] = s0; red[" << NT << " + t] = s1; red[" << 2 * NT << " + t] = s2;\n";
+        o << "    __syncthreads();\n";
+        o << "    for (unsigned h = " << NT / 2 << "u; h > 0u; h >>= 1) {\n";
+        o << "      if (t < h) { red[t] += red[t + h]; red[" << NT << " + t] += red[" << NT << " + t + h]; red[" << 2 * NT
+          << " + t] += red[" << 2 * NT << " + t + h]; }\n";
+        o << "      __syncthreads();\n    }\n";
+        o << "    if (t == 0) { part[3 * b] = red[0]; part[3 * b + 1] = red[" << NT << "]; part[3 * b + 2] = red["
+          << 2 * NT << "]; }\n";
+        o << "    }\n";
     }
 };
 
@@ -631,10 +692,10 @@ hq_status compile_source(const std::string& src, std::vector<char>* code, std::s
 
 // Source of the specialised kernel; empty (and the error set) if the plan
 // cannot be emitted.
-std::string jit_source(const ShmPlan& plan, std::vector<double>* coef)
+std::string jit_source(const ShmPlan& plan, std::vector<double>* coef, bool batch, bool readout)
 {
     try {
-        Emitter e(plan, coef);
+        Emitter e(plan, coef, batch, readout);
         return std::string(kSrcHeader) + e.emit();
     } catch (const std::exception& ex) {
         set_error(std::string("specialised kernel: ") + ex.what());
@@ -660,6 +721,25 @@ static hq_status load_program(const ShmPlan& plan, const std::vector<char>& code
             std::vector<float> f(coef.begin(), coef.end());
             HQ_HIP(hipMemcpy(p->d_coef, f.data(), f.size() * 4, hipMemcpyHostToDevice));
         }
+    }
+    return HQ_OK;
+}
+
+hq_status jit_compile_load(const std::string& src, const char* name, hipModule_t* mod, hipFunction_t* fn)
+{
+    *mod = nullptr;
+    *fn = nullptr;
+    std::vector<char> code;
+    std::string err;
+    hq_status st = compile_source(src, &code, &err);
+    if (st != HQ_OK) { set_error(err); return st; }
+    HQ_HIP(hipModuleLoadData(mod, code.data()));
+    hipError_t e = hipModuleGetFunction(fn, *mod, name);
+    if (e != hipSuccess) {
+        (void)hipModuleUnload(*mod);
+        *mod = nullptr;
+        set_error(std::string("hipModuleGetFunction(") + name + "): " + hipGetErrorString(e));
+        return HQ_ERR_HIP;
     }
     return HQ_OK;
 }
@@ -735,16 +815,18 @@ extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gat
 {
     HQ_CHECK(n >= 1 && n <= 36, HQ_ERR_ARG, "hq_sharemem_source: num_qubits must be in [1, 36]");
     HQ_CHECK(precision == HQ_C128 || precision == HQ_C64, HQ_ERR_ARG, "hq_sharemem_source: bad precision");
-    HQ_CHECK(variant >= 0 && variant <= 3, HQ_ERR_ARG, "hq_sharemem_source: variant must be in [0, 3]");
-    const bool whole_line = variant & 1, init = (variant & 2) != 0;
+    HQ_CHECK(variant >= 0 && variant <= 31, HQ_ERR_ARG, "hq_sharemem_source: variant must be in [0, 31]");
+    const bool whole_line = variant & 1, init = (variant & 2) != 0, batch = (variant & 4) != 0,
+               readout = (variant & 8) != 0, shared_code = (variant & 16) != 0;
+    HQ_CHECK(batch || !readout, HQ_ERR_ARG, "hq_sharemem_source: the readout fold needs the batched form");
     hq::ShmPlan plan;
     hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan, init,
-                                     whole_line ? 3 : 0);
+                                     whole_line ? 3 : 0, shared_code);
     if (st != HQ_OK) return st;
     plan.nt = whole_line;
     plan.init = init;
     std::vector<double> cf;
-    std::string src = hq::jit_source(plan, &cf);
+    std::string src = hq::jit_source(plan, &cf, batch, readout);
     if (src.empty()) return HQ_ERR_UNSUPPORTED;
     if (len) *len = src.size();
     if (buf && cap) {

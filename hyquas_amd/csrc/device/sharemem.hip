@@ -110,9 +110,14 @@ struct Lowered {
 // = c [[1, -t], [t, 1]] (t = s/c) or s [[u, -1], [1, u]] (u = c/s), whichever
 // keeps |t|, |u| <= 1; RX-type [[a, ib], [ib, a]] likewise; a unit diagonal
 // diag(d0, d1) = d0 diag(1, d1/d0).  Other gates are left as they are.
-void factor_scalar(Lowered& L, std::complex<double>* S)
+void factor_scalar(Lowered& L, std::complex<double>* S, bool shared_code)
 {
     double* m = L.m;
+    // shared_code: only factorings whose op class does not depend on the
+    // angle (H, unit diagonals), so groups that differ only in angles emit the
+    // same specialised kernel source
+    if (shared_code && (L.cls == SHM_RX || (L.cls == SHM_REAL && !(m[0] == m[2] && m[0] == m[4] && m[0] == -m[6]))))
+        return;
     if (L.cls == SHM_REAL) {
         if (m[0] == m[2] && m[0] == m[4] && m[0] == -m[6] && m[0] != 0) {
             *S *= m[0];
@@ -169,7 +174,7 @@ int rank3(const std::vector<uint32_t>& v)
 }  // namespace
 
 hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gates, uint64_t shared_mask, int rb_req,
-                        ShmPlan* plan, bool init, int io)
+                        ShmPlan* plan, bool init, int io, bool shared_code)
 {
     HQ_CHECK(num_gates >= 0 && (num_gates == 0 || gates), HQ_ERR_ARG, "sharemem: bad gate array");
     const size_t amp_bytes = precision == HQ_C128 ? 16 : 8;
@@ -216,7 +221,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
             L.c_block[c] = cs[c] >= 0 ? block_of[cs[c]] : -1;
         }
         memcpy(L.m, g.mat, sizeof(L.m));
-        if (factor && g.control < 0 && g.control2 < 0) factor_scalar(L, &S);
+        if (factor && g.control < 0 && g.control2 < 0) factor_scalar(L, &S, shared_code);
         low.push_back(L);
         low_type.push_back(g.type);
     }

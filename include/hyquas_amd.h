@@ -179,8 +179,14 @@ hq_status hq_program_is_specialized(hq_program p, int *yes);
  * 1 the whole-line non-temporal one (stages without gates added where the
  * first or last stage would access HBM in pieces shorter than 128 bytes),
  * the two kernels the "tune" option chooses between; bit 1: the kernel of a
- * run's first group, which builds |0...0> instead of loading the state.
- * For inspection, caching and CPU tests. */
+ * run's first group, which builds |0...0> instead of loading the state;
+ * bit 2: the batched form of the engine, hq_group_batch(items, count), whose
+ * blocks map to (circuit, tile) through an item table (40-byte items:
+ * state, coefficients, partials, first block, qubits, mode; mode bit 1 = no
+ * store); bit 3 (with bit 2): the circuit's last group, which also writes
+ * each tile's sum p, sum p Z and sum p ZZ to the item's partials; bit 4:
+ * op classes independent of the gate angles (same-structure groups share
+ * one source).  For inspection, caching and CPU tests. */
 hq_status hq_sharemem_source(int num_qubits, int precision, const hq_gate *gates, int num_gates,
                              uint64_t shared_mask, int reg_qubits, int variant, char *buf, size_t cap,
                              size_t *len, double *coef, size_t coef_cap, size_t *num_coef, int *shape);
@@ -390,6 +396,11 @@ hq_status hq_engine_destroy(hq_engine e);
 /* "whole_max" (13), "share_qubits" (12), "coalesce" (4), "compile_threads" (4,
  * before the first submit), "max_inflight" (2 rounds queued ahead of the host). */
 hq_status hq_engine_set_option(hq_engine e, const char *key, int64_t value);
+/* Specialised batched kernels the engine has compiled so far (option "jit";
+ * one per distinct group structure).  Safe to call while another thread
+ * compiles circuits into the engine (hq_batch_create, hq_engine_submit). */
+hq_status hq_engine_kernels(hq_engine e, int64_t *compiled);
+
 /* Tile qubits for circuits of num_qubits qubits (0: the whole_max/share_qubits
  * rule); hq_engine_get_tile reports the tile in effect. */
 hq_status hq_engine_set_tile(hq_engine e, int num_qubits, int tile);

@@ -161,8 +161,8 @@ def rand_state(n, seed, dtype=np.complex128):
     return (v / np.linalg.norm(v)).astype(dtype)
 
 
-def group_for_mask(n, mask, m, seed):
-    """Random gates (all types) whose non-diagonal targets lie in the tile `mask`."""
+def group_spec_for_mask(n, mask, m, seed):
+    """Random gates (all types, as tuples) whose non-diagonal targets lie in the tile `mask`."""
     rs = np.random.default_rng(seed)
     tile = [q for q in range(n) if (mask >> q) & 1]
     spec = []
@@ -172,7 +172,23 @@ def group_for_mask(n, mask, m, seed):
             new_t = int(rs.choice([q for q in tile if q != c and (len(s) < 5 or q != s[4])]))
             s = (t, new_t) + tuple(s[2:])
         spec.append(s)
-    return to_oracle(spec)
+    return spec
+
+
+def group_for_mask(n, mask, m, seed):
+    """Random gates (all types) whose non-diagonal targets lie in the tile `mask`."""
+    return to_oracle(group_spec_for_mask(n, mask, m, seed))
+
+
+def reangle(spec, seed):
+    """The same gates with every angle drawn again (same structure)."""
+    rs = np.random.default_rng(seed)
+    out = []
+    for s in spec:
+        params = s[3] if len(s) > 3 else ()
+        s = (s[0], s[1], s[2], tuple(float(x) for x in rs.uniform(-math.pi, math.pi, len(params)))) + tuple(s[4:])
+        out.append(s)
+    return out
 
 
 _QASM_NAMES = {CCX: "ccx", CNOT: "cx", CY: "cy", CZ: "cz", CRX: "crx", CRY: "cry", CRZ: "crz", CU1: "cu1",

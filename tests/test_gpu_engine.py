@@ -241,3 +241,64 @@ def test_state_sample_basis_state(gpu):
     s.upload(psi)
     assert set(s.sample(100, 3).tolist()) == {12345}
     s.close()
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_specialised_batch_kernels(gpu, prec):
+    """Option "jit": the engine's groups run as specialised batched kernels,
+    shared by groups of the same structure (angles in per-circuit
+    coefficients).  A batch of ansatz circuits with repeated (qubits, depth)
+    and random-layer circuits, with readout and top-k, against the oracle;
+    the same results alone and in a mix; graph replays repeat bit for bit."""
+    hq = gpu
+    p = hq.HQ_C128 if prec == "c128" else hq.HQ_C64
+    tol = 1e-10 if prec == "c128" else 2e-5
+    specs = [("hea", 16, 6, 1), ("hea", 16, 6, 2), ("hea", 16, 6, 3), ("hea", 19, 4, 4), ("hea", 11, 8, 5),
+             ("hea", 19, 4, 6), ("rl", 17, 5, 7), ("rl", 13, 9, 8), ("hea", 12, 3, 9), ("rl", 20, 3, 10)]
+
+    def build(kind, n, d, s):
+        return _hea(hq, n, d, s) if kind == "hea" else _rl(hq, n, d, s)
+
+    e = hq.Engine(p, 1 << 31)
+    e.set_option("jit", 1)
+    e.set_option("compile_threads", 8)
+    b = e.batch([build(*s) for s in specs], hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+    b.run()
+    r1, t1 = b.results()
+    for i, (kind, n, d, s) in enumerate(specs):
+        spec = hea_layers(n, d, s) if kind == "hea" else random_layers(n, d, s)
+        psi = oracle.run_plain(to_oracle(spec), n)
+        nrm, z, zz = _readouts(psi, n)
+        print(i, kind, n, d, r1[i], (nrm, z, zz), flush=True)
+        assert abs(r1[i][0] - nrm) <= tol and abs(r1[i][1] - z) <= tol * n and abs(r1[i][2] - zz) <= tol * n
+        idx, prob = _topk_ref(psi)
+        np.testing.assert_allclose(t1[1][i], prob, rtol=0, atol=tol)
+        # the indices carry those probabilities (ties may come in either order)
+        got_idx = np.array([int(x) for x in t1[0][i]])
+        np.testing.assert_allclose(np.abs(psi[got_idx]) ** 2, prob, rtol=0, atol=tol)
+    for _ in range(2):
+        b.run()
+        r2, t2 = b.results()
+        assert np.array_equal(r1, r2) and np.array_equal(t1[1], t2[1])
+    b.close()
+    # continuous batching with the same kernels: alone and in a mix
+    alone = {}
+    for i in (0, 3, 6):
+        e1 = hq.Engine(p, 1 << 31)
+        e1.set_option("jit", 1)
+        j = e1.submit(build(*specs[i]), hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+        e1.drain()
+        alone[i] = e1.result(j, topk=True)
+        e1.close()
+    jobs = {}
+    for i, s in enumerate(specs):
+        jobs[i] = e.submit(build(*s), hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+        if i % 3 == 0:
+            e.step()
+    e.drain()
+    for i in (0, 3, 6):
+        r = e.result(jobs[i], topk=True)
+        a = alone[i]
+        assert (r["norm"], r["z"], r["zz"]) == (a["norm"], a["z"], a["zz"])
+        assert r["topk"] == a["topk"]
+    e.close()

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import group_for_mask as _group_for_mask, rand_state as _rand_state, rel_err
+from helpers import group_for_mask as _group_for_mask, rand_state as _rand_state, rel_err, to_oracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -85,3 +85,90 @@ def test_split_exchange_for_large_tiles(hq):
     assert "__shared__ C lds[4096]" in src and " q0 = lds[" in src
     src64, _, _ = hq.sharemem_source(n, g, mask, 1, 5)    # complex64: 64 KiB, one exchange phase
     assert "__shared__ C lds[8192]" in src64 and " q0 = lds[" not in src64
+
+
+def _zsums(n):
+    idx = np.arange(1 << n)
+    z = np.zeros(1 << n)
+    zz = np.zeros(1 << n)
+    for q in range(n):
+        z += 1 - 2 * ((idx >> q) & 1)
+    for q in range(n - 1):
+        zz += 1 - 2 * (((idx >> q) ^ (idx >> (q + 1))) & 1)
+    return z, zz
+
+
+@pytest.mark.parametrize("n,k,seed,prec,rb,variant", [
+    (12, 10, 51, 0, 0, 4 | 16), (13, 11, 52, 1, 0, 4 | 8 | 16), (14, 13, 53, 0, 5, 4 | 8 | 16),
+    (14, 13, 54, 0, 5, 2 | 4 | 8 | 16), (15, 14, 55, 1, 6, 1 | 4 | 8 | 16), (12, 12, 56, 0, 4, 2 | 4 | 8 | 16)])
+def test_batched_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, variant):
+    """The engine's batched form of a generated group kernel (item table,
+    shared code for same-structure groups, fused readout, first-group form),
+    run on the CPU for three circuits of one structure with their own angles:
+    one source for all three, each state equal to the oracle's, each tile's
+    partials summing to the circuit's norm, sum <Z_i> and sum <Z_i Z_i+1>."""
+    from helpers import group_spec_for_mask, reangle
+    rs = np.random.default_rng(seed)
+    rest = rs.choice(np.arange(3, n), k - 3, replace=False) if k < n else np.arange(3, n)
+    mask = 0
+    for q in [0, 1, 2] + [int(x) for x in rest]:
+        mask |= 1 << q
+    base_spec = group_spec_for_mask(n, mask, 90, seed)
+    specs = [reangle(base_spec, seed + 100 * i) for i in range(3)]
+    srcs, coefs = [], []
+    for sp in specs:
+        src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, to_oracle(sp), mask, prec, rb, variant)
+        srcs.append(src)
+        coefs.append(np.asarray(coef, dtype=np.float64))
+    assert srcs[0] == srcs[1] == srcs[2], "same structure must give one kernel source"
+    assert "hq_group_batch" in srcs[0]
+    init, readout = bool(variant & 2), bool(variant & 8)
+    cxx = CLANG if os.path.exists(CLANG) else "g++"
+    if cxx == "g++" and "ext_vector_type" in srcs[0]:
+        pytest.skip("needs ROCm clang++ for clang vector extensions")
+    ksrc = tmp_path / "kernel.hip"
+    ksrc.write_text(srcs[0])
+    exe = tmp_path / "harness"
+    subprocess.run([cxx, "-O1", "-std=c++20", "-pthread", "-Wno-unknown-attributes", f'-DGROUP_SRC="{ksrc}"',
+                    os.path.join(HERE, "cpp", "jit_batch_harness.cpp"), "-o", str(exe)], check=True)
+    dt = np.complex128 if prec == 0 else np.complex64
+    psi0 = [rand_state_or_zero(n, seed + i, init) for i in range(3)]
+    np.concatenate(psi0).astype(dt).tofile(tmp_path / "in.bin")
+    np.concatenate(coefs).tofile(tmp_path / "coef.bin")
+    subprocess.run([str(exe), str(n), str(k), str(1 << tbits), "3", "0", str(tmp_path / "in.bin"),
+                    str(tmp_path / "coef.bin"), str(tmp_path / "out.bin"), str(tmp_path / "part.bin")],
+                   check=True, timeout=300)
+    got = np.fromfile(tmp_path / "out.bin", dtype=dt).reshape(3, -1)
+    part = np.fromfile(tmp_path / "part.bin", dtype=np.float64).reshape(3, -1, 3)
+    z, zz = _zsums(n)
+    for i in range(3):
+        start = np.zeros(1 << n, dtype=np.complex128)
+        if init:
+            start[0] = 1
+        else:
+            start = psi0[i]
+        want = oracle.run_plain(to_oracle(specs[i]), n, start)
+        assert rel_err(got[i], want) <= (1e-13 if prec == 0 else 2e-6)
+        if readout:
+            p = np.abs(want) ** 2
+            tot = part[i].sum(axis=0)
+            tol = 1e-12 if prec == 0 else 1e-5
+            assert abs(tot[0] - p.sum()) <= tol
+            assert abs(tot[1] - np.dot(p, z)) <= tol * n
+            assert abs(tot[2] - np.dot(p, zz)) <= tol * n
+    # mode bit 1: no store (the partials still come out)
+    if readout:
+        np.concatenate(psi0).astype(dt).tofile(tmp_path / "in.bin")
+        subprocess.run([str(exe), str(n), str(k), str(1 << tbits), "3", "2", str(tmp_path / "in.bin"),
+                        str(tmp_path / "coef.bin"), str(tmp_path / "out2.bin"), str(tmp_path / "part2.bin")],
+                       check=True, timeout=300)
+        same = np.fromfile(tmp_path / "out2.bin", dtype=dt).reshape(3, -1)
+        assert np.array_equal(same, np.concatenate(psi0).astype(dt).reshape(3, -1))
+        assert np.array_equal(np.fromfile(tmp_path / "part2.bin").reshape(3, -1, 3), part)
+
+
+def rand_state_or_zero(n, seed, init):
+    # a first-group kernel never reads the state: hand it garbage
+    if init:
+        return np.full(1 << n, 7.0 + 3.0j)
+    return _rand_state(n, seed)
