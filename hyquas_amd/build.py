@@ -1,5 +1,6 @@
-"""Build the native library ``hyquas_amd/libhyquas_amd.so`` and the CLI
-``hyquas_amd/hyquas_main`` in-tree.
+"""Build the native library ``hyquas_amd/libhyquas_amd.so``, the CLI
+``hyquas_amd/hyquas_main`` and the kernel compile helper ``hyquas_amd/hq_rtc``
+in-tree.
 
 Device sources (``csrc/device/*.hip``) are compiled by hipcc for gfx950 only;
 host sources (``csrc/host/*.cpp``) by g++ with OpenMP; everything is linked
@@ -22,6 +23,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhyquas_amd.so")
 CLI = os.path.join(HERE, "hyquas_main")
+RTC = os.path.join(HERE, "hq_rtc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -72,11 +75,17 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         cmds.append([CXX] + CXX_FLAGS + ["-c", s, "-o", _obj(s)])
     main_src = os.path.join(CSRC, "host", "main.cpp")
     cmds.append([CXX] + CXX_FLAGS + ["-c", main_src, "-o", _obj(main_src)])
+    # the compile helper: plain C++ over hiprtc, no GPU code
+    cmds.append([CXX, "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
+                 "-I" + os.path.join(CSRC, "device"), os.path.join(CSRC, "rtc", "hq_rtc.cpp"),
+                 "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
+                 "-o", RTC + ".tmp"])
     jobs = jobs or min(8, os.cpu_count() or 1)
     # longest first: the OShareMem kernels dominate
     cmds.sort(key=lambda c: 0 if "sharemem" in c[-3] else 1)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
+    os.replace(RTC + ".tmp", RTC)
     objs = [_obj(s) for s in dev + host]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +

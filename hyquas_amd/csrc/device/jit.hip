@@ -8,24 +8,36 @@
 // exchange addresses are literals; X/CNOT become register renaming; the
 // angles of parametric gates stay runtime data (a coefficient array), so
 // circuits that differ only in angles share compiled code.  Sources are
-// compiled with hiprtc for gfx950 in parallel and cached in memory and on
-// disk (HQ_JIT_CACHE, default ~/.cache/hyquas_amd/jit).
+// compiled with hiprtc for gfx950 and cached in memory and on disk
+// (HQ_JIT_CACHE, default ~/.cache/hyquas_amd/jit).  hiprtc compiles one
+// program at a time per process, so concurrent compiles run as child
+// processes of the helper hq_rtc next to the library (HQ_RTC_HELPER names
+// another path, or "none" to compile in this process).
 
 #include "hq_ops.h"
+#include "jit_opts.h"
 
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <fstream>
 #include <mutex>
+#include <spawn.h>
 #include <sstream>
 #include <stdexcept>
 #include <sys/stat.h>
+#include <sys/wait.h>
 #include <thread>
+#include <unistd.h>
 #include <unordered_map>
+
+extern char** environ;
 
 namespace hq {
 
@@ -597,11 +609,6 @@ bool read_file(const std::string& p, std::string* out)
 
 const char* kSrcHeader = "// hyquas_amd group kernel v1\n";
 
-// hiprtc options of every specialised kernel
-const char* const kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=fast", "-Xclang", "-target-feature",
-                                "-Xclang", "-packed-fp32-ops"};
-constexpr int kNumJitOpts = (int)(sizeof(kJitOpts) / sizeof(kJitOpts[0]));
-
 // What besides the source decides the code object: the options and the
 // hiprtc version.  Part of the disk cache key, so a flag or toolchain change
 // never serves a code object built under other settings.
@@ -627,6 +634,85 @@ bool write_file(const std::string& p, const char* data, size_t n)
     return !f.fail();
 }
 
+// The compile helper: $HQ_RTC_HELPER, else hq_rtc in the library's folder;
+// empty when there is none ("none" asks for in-process compiles).
+const std::string& helper_path()
+{
+    static const std::string path = [] {
+        const char* e = getenv("HQ_RTC_HELPER");
+        if (e && *e) return strcmp(e, "none") == 0 ? std::string() : std::string(e);
+        Dl_info info;
+        if (dladdr((void*)&helper_path, &info) && info.dli_fname) {
+            const std::string lib = info.dli_fname;
+            const size_t slash = lib.rfind('/');
+            const std::string h = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/hq_rtc";
+            if (access(h.c_str(), X_OK) == 0) return h;
+        }
+        return std::string();
+    }();
+    return path;
+}
+
+std::atomic<bool> g_helper_failed{false};
+
+// Compile `src` in a child process of the helper.  1: compiled; -1: the
+// source does not compile (err holds the log); 0: no helper or it could not
+// run (the caller compiles in this process).
+int helper_compile(const std::string& src, std::vector<char>* code, std::string* err)
+{
+    const std::string& h = helper_path();
+    if (h.empty() || g_helper_failed.load()) return 0;
+    static std::atomic<unsigned long long> counter{0};
+    const char* t = getenv("TMPDIR");
+    const std::string base = std::string(t && *t ? t : "/tmp") + "/hq_rtc_" + std::to_string((long long)getpid()) +
+                             "_" + std::to_string(counter++);
+    const std::string fin = base + ".hip", fout = base + ".co", flog = base + ".log";
+    if (!write_file(fin, src.data(), src.size())) return 0;
+    auto cleanup = [&] {
+        (void)remove(fin.c_str());
+        (void)remove(fout.c_str());
+        (void)remove(flog.c_str());
+    };
+    posix_spawn_file_actions_t fa;
+    posix_spawn_file_actions_init(&fa);
+    posix_spawn_file_actions_addclosefrom_np(&fa, 3);   // the child inherits no device or file handles
+    std::vector<char*> argv = {const_cast<char*>(h.c_str()), const_cast<char*>(fin.c_str()),
+                               const_cast<char*>(fout.c_str()), const_cast<char*>(flog.c_str()), nullptr};
+    pid_t pid = -1;
+    const int rc = posix_spawn(&pid, h.c_str(), &fa, nullptr, argv.data(), environ);
+    posix_spawn_file_actions_destroy(&fa);
+    if (rc != 0) {
+        cleanup();
+        if (!g_helper_failed.exchange(true))
+            fprintf(stderr, "hyquas_amd: compile helper %s could not start (%s); compiling in-process\n", h.c_str(),
+                    strerror(rc));
+        return 0;
+    }
+    int status = 0;
+    pid_t w;
+    do { w = waitpid(pid, &status, 0); } while (w < 0 && errno == EINTR);
+    const int code_rc = (w == pid && WIFEXITED(status)) ? WEXITSTATUS(status) : -1;
+    if (code_rc == 0) {
+        std::string c;
+        if (read_file(fout, &c) && !c.empty()) {
+            code->assign(c.begin(), c.end());
+            cleanup();
+            return 1;
+        }
+    } else if (code_rc == 1) {
+        std::string log;
+        (void)read_file(flog, &log);
+        *err = "hiprtc: " + log.substr(0, 2000);
+        cleanup();
+        return -1;
+    }
+    cleanup();
+    if (!g_helper_failed.exchange(true))
+        fprintf(stderr, "hyquas_amd: compile helper %s failed (status %d); compiling in-process\n", h.c_str(),
+                code_rc);
+    return 0;
+}
+
 hq_status compile_source(const std::string& src, std::vector<char>* code, std::string* err)
 {
     {
@@ -648,26 +734,30 @@ hq_status compile_source(const std::string& src, std::vector<char>* code, std::s
             return HQ_OK;
         }
     }
-    hiprtcProgram prog;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "hq_group.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
-        *err = "hiprtcCreateProgram failed";
-        return HQ_ERR_HIP;
-    }
-    hiprtcResult rc = hiprtcCompileProgram(prog, kNumJitOpts, kJitOpts);
-    if (rc != HIPRTC_SUCCESS) {
-        size_t n = 0;
-        hiprtcGetProgramLogSize(prog, &n);
-        std::string log(n, '\0');
-        if (n) hiprtcGetProgramLog(prog, &log[0]);
-        *err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + "\n" + log.substr(0, 2000);
+    const int via = helper_compile(src, code, err);
+    if (via < 0) return HQ_ERR_HIP;
+    if (via == 0) {
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "hq_group.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+            *err = "hiprtcCreateProgram failed";
+            return HQ_ERR_HIP;
+        }
+        hiprtcResult rc = hiprtcCompileProgram(prog, kNumJitOpts, kJitOpts);
+        if (rc != HIPRTC_SUCCESS) {
+            size_t n = 0;
+            hiprtcGetProgramLogSize(prog, &n);
+            std::string log(n, '\0');
+            if (n) hiprtcGetProgramLog(prog, &log[0]);
+            *err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + "\n" + log.substr(0, 2000);
+            hiprtcDestroyProgram(&prog);
+            return HQ_ERR_HIP;
+        }
+        size_t sz = 0;
+        hiprtcGetCodeSize(prog, &sz);
+        code->resize(sz);
+        hiprtcGetCode(prog, code->data());
         hiprtcDestroyProgram(&prog);
-        return HQ_ERR_HIP;
     }
-    size_t sz = 0;
-    hiprtcGetCodeSize(prog, &sz);
-    code->resize(sz);
-    hiprtcGetCode(prog, code->data());
-    hiprtcDestroyProgram(&prog);
     if (!dir.empty()) {
         // write both files under temporary names, then rename; a failed or
         // short write leaves nothing behind (the disk cache is best effort)
@@ -756,7 +846,8 @@ hq_status jit_sharemem(const ShmPlan& plan, hq_program p)
     return load_program(plan, code, coef, p);
 }
 
-// Compile many plans in parallel (hiprtc programs are independent).
+// Compile many plans in parallel (child processes of the compile helper;
+// in this process hiprtc runs them one at a time).
 hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std::vector<hq_program>& progs,
                             int threads)
 {
@@ -770,10 +861,11 @@ hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std:
     std::vector<std::vector<char>> codes(n);
     std::vector<hq_status> sts(n, HQ_OK);
     std::vector<std::string> errs(n);
-    // warm the compiler once before fanning out
-    if (n > 0) sts[0] = compile_source(srcs[0], &codes[0], &errs[0]);
+    // in-process compiles: warm the compiler once before fanning out
+    const bool warm = helper_path().empty();
+    if (n > 0 && warm) sts[0] = compile_source(srcs[0], &codes[0], &errs[0]);
     std::mutex mu;
-    size_t next = 1;
+    size_t next = warm ? 1 : 0;
     auto worker = [&]() {
         while (true) {
             size_t i;
@@ -785,9 +877,9 @@ hq_status jit_sharemem_many(const std::vector<const ShmPlan*>& plans, const std:
             sts[i] = compile_source(srcs[i], &codes[i], &errs[i]);
         }
     };
-    int nt = std::max(1, std::min<int>(threads, (int)n - 1));
+    const int nt = std::max(1, std::min<int>(threads, (int)(n - next)));
     std::vector<std::thread> pool;
-    for (int i = 0; i < nt && n > 1; i++) pool.emplace_back(worker);
+    for (int i = 0; i < nt && next < n; i++) pool.emplace_back(worker);
     for (auto& th : pool) th.join();
     for (size_t i = 0; i < n; i++) {
         if (sts[i] != HQ_OK) { set_error(errs[i]); return sts[i]; }
