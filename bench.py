@@ -416,11 +416,18 @@ def batch_form(a, hq, count, plain_sample, precisions=("c64", "c128")):
         ex, _ = b.results()
         zz[name] = ex[:, 2].copy()
         info = b.info()
+        dev_ms = float(np.mean(dev))
+        # the replay's tile passes against HBM: algorithmic bytes (each pass's
+        # live tiles read and written once) over the replay's device time
+        gbps = info["pass_bytes"] / (dev_ms * 1e-3) / 1e9
         res[name] = {"kernels": "specialised" if a.batch_jit else "interpreter",
                      "circuits_per_s": round(count / wall, 1), "s_per_batch": round(wall, 6),
                      "device_ms": round(min(dev), 3), "rounds": info["rounds"], "launches": info["launches"],
                      "device_bytes": info["device_bytes"], "build_s": round(build_s, 3),
                      "autotune_s": round(tune_s, 3), "tiles": tiles,
+                     "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 4),
+                                  "pass_bytes": info["pass_bytes"], "device_ms_mean": round(dev_ms, 3)},
                      "max_norm_error": float(np.max(np.abs(ex[:, 0] - 1)))}
         b.close()
         e.close()

@@ -151,6 +151,7 @@ def _declare(L):
         "hq_batch_run": (S, [c_void_p, POINTER(c_double)]),
         "hq_batch_results": (S, [c_void_p, POINTER(c_double), POINTER(c_uint64), POINTER(c_double)]),
         "hq_batch_info": (S, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_uint64)]),
+        "hq_batch_pass_bytes": (S, [c_void_p, POINTER(c_uint64)]),
         "hq_batch_destroy": (S, [c_void_p]),
         "hq_state_topk": (S, [c_void_p, c_int, POINTER(c_uint64), POINTER(c_double)]),
         "hq_state_sample": (S, [c_void_p, c_int, c_uint64, POINTER(c_uint64)]),
@@ -785,7 +786,9 @@ class Batch:
         c, r, l = c_int(), c_int(), c_int()
         b = c_uint64()
         _check(lib().hq_batch_info(self.handle, byref(c), byref(r), byref(l), byref(b)))
-        return dict(count=c.value, rounds=r.value, launches=l.value, device_bytes=b.value)
+        p = c_uint64()
+        _check(lib().hq_batch_pass_bytes(self.handle, byref(p)))
+        return dict(count=c.value, rounds=r.value, launches=l.value, device_bytes=b.value, pass_bytes=p.value)
 
     def results(self):
         """(expect[count, 3] = norm, sum Z, sum ZZ; topk (idx[count, 16], prob[count, 16]) or None)."""

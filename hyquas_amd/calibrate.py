@@ -210,12 +210,20 @@ def main(argv=None):
     os.makedirs(a.out, exist_ok=True)
     n = a.qubits
     t0 = time.perf_counter()
+
+    def done(what):
+        print(f"calibrate: {what} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
     bw64 = run_length_table(n, hq.HQ_C128, a.samples)
+    done("complex128 run-length table")
     bw32 = run_length_table(n, hq.HQ_C64, a.samples)
+    done("complex64 run-length table")
     v64 = valu_rate(n, hq.HQ_C128)
     v32 = valu_rate(n, hq.HQ_C64)
+    done("VALU rates")
     r64, _, t64, tbw = dense_and_transpose(n - 1, hq.HQ_C128)
     r32, _, _, _ = dense_and_transpose(n - 1, hq.HQ_C64)
+    done("dense GEMM and transpose")
     lo = launch_overhead()
     params = os.path.join(a.out, "mi355x.params")
     with open(params, "w") as f:
@@ -225,6 +233,7 @@ def main(argv=None):
         path = os.path.join(a.out, f"{m}qubits.out")
         reference_tables(m, hq.HQ_C64 if a.c64 else hq.HQ_C128, path)
         tables.append(path)
+        done(f"reference-format table for {m} qubits")
     print(json.dumps({"params": params, "tables": tables, "qubits": n, "seconds": round(time.perf_counter() - t0, 1),
                       "shm_bw_run_c128": [round(b / 1e12, 3) for b in bw64],
                       "shm_bw_run_c64": [round(b / 1e12, 3) for b in bw32],

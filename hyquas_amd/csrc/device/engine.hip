@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -357,6 +358,7 @@ struct GroupPlan {
     uint32_t nops = 0;
     uint64_t blob_off = 0;        // the interpreter's program, or the specialised kernel's coefficients
     hipFunction_t fn = nullptr;   // specialised batched kernel (null: the interpreter runs the group)
+    double live = 1.0;            // share of one read + write of the state the pass moves
 };
 
 // Specialised batched kernels of an engine, one per distinct source: groups
@@ -481,6 +483,9 @@ static hq_status add_group(JobPlan* jp, int n, int precision, const std::vector<
     hq_status st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), mask, 0, &plan, kc && first, 0, kc != nullptr);
     if (st != HQ_OK) return st;
     plan.hdr.zero_mask = zero_mask;
+    // the first pass only writes; later ones skip the tiles whose block
+    // address sets a qubit still known to be |0>
+    const double live = first ? 0.5 : std::ldexp(1.0, -__builtin_popcountll(zero_mask & ~mask));
     // groups of more than kJitMaxOps ops stay with the interpreter (compile time)
     constexpr size_t kJitMaxOps = 512;
     if (kc && plan.ops.size() <= kJitMaxOps) {
@@ -490,11 +495,13 @@ static hq_status add_group(JobPlan* jp, int n, int precision, const std::vector<
         hipFunction_t fn = nullptr;
         if (!src.empty() && kc->get(src, &fn) == HQ_OK && fn) {
             append_jit(jp, plan, coef, fn);
+            jp->groups.back().live = live;
             return HQ_OK;
         }
         // the interpreter takes the group when its kernel cannot be built
     }
     append_blob(jp, plan);
+    jp->groups.back().live = live;
     return HQ_OK;
 }
 
@@ -1474,6 +1481,17 @@ hq_status hq_batch_info(hq_batch b, int* count, int* rounds, int* launches, uint
         for (auto& j : b->jobs) for (auto& x : j->blocks) t += 1ull << x.order;
         *device_bytes = t;
     }
+    return HQ_OK;
+}
+
+hq_status hq_batch_pass_bytes(hq_batch b, uint64_t* bytes)
+{
+    HQ_CHECK(b && bytes, HQ_ERR_ARG, "hq_batch_pass_bytes: null argument");
+    const double amp = b->e->precision == HQ_C128 ? 16.0 : 8.0;
+    double t = 0;
+    for (auto& j : b->jobs)
+        for (auto& g : j->plan.groups) t += 2.0 * amp * std::ldexp(1.0, j->plan.n) * g.live;
+    *bytes = (uint64_t)t;
     return HQ_OK;
 }
 

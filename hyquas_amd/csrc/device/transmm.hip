@@ -238,8 +238,14 @@ template <> __device__ __forceinline__ int mfma_row<double>(int lane, int v) { r
 // method): with T1 = Ar Br, T2 = Ai Bi and T3 = (Ar + Ai)(Br + Bi) summed
 // over k, Re C = T1 - T2 and Im C = T3 - T1 - T2.  A quarter fewer matrix-core
 // operations for the same product; the sums Ar + Ai are formed once per
-// kernel, Br + Bi once per B element in LDS.  The extra rounding is one
+// kernel, Br + Bi once per B element read.  The extra rounding is one
 // addition per operand and one per output (FP64: ~1e-16 relative).
+//
+// Pipeline: two LDS buffers.  A block loads its next tile's B from HBM into
+// registers before the matrix-core loop of the current tile and stages it
+// into the other buffer afterwards, so the load's latency hides behind the
+// MFMAs; the C tile goes back through the current buffer for coalesced
+// stores.  Two barriers per tile.
 template <typename R, int MQ, int NT, bool GAUSS>
 __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx<R>* __restrict__ in,
                                                                      cplx<R>* __restrict__ out,
@@ -250,13 +256,15 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
     constexpr int KS = K / 4;          // k-steps
     constexpr int NCB = NT / 16;       // column blocks per tile
     constexpr int NTP = NT + 1;        // padded plane row
+    constexpr int NTH = 64 * (K / 16); // threads per block
+    constexpr int PER = NT * K / NTH;  // B elements per thread per tile
+    static_assert(NT * K % NTH == 0, "tile must split evenly over the block");
     using Acc = typename MfmaAcc<R>::T;
-    __shared__ R bre[K * NTP];
-    __shared__ R bim[K * NTP];
+    __shared__ R bre[2][K * NTP];
+    __shared__ R bim[2][K * NTP];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nthreads = blockDim.x;
     const int arow = wave * 16 + (lane & 15);
     const int kq = lane >> 4;          // 0..3
 
@@ -271,16 +279,30 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
     }
 
     const uint64_t ntiles = ncols / NT;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const cplx<R>* __restrict__ src = in + tile * NT * K;
-        // stage B (K x NT) as planes bre[k][col]
-        for (int e = threadIdx.x; e < NT * K; e += nthreads) {
-            cplx<R> v = src[e];
-            int col = e / K, k = e % K;
-            bre[k * NTP + col] = v.x;
-            bim[k * NTP + col] = v.y;
+    uint64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;         // uniform: the grid never exceeds the tiles
+    cplx<R> pf[PER];
+    auto load = [&](uint64_t t) {
+        const cplx<R>* __restrict__ src = in + t * NT * K;
+#pragma unroll
+        for (int i = 0; i < PER; i++) pf[i] = src[threadIdx.x + i * NTH];
+    };
+    // B (K x NT) as planes [k][col]
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int e = threadIdx.x + i * NTH;
+            const int col = e / K, k = e % K;
+            bre[buf][k * NTP + col] = pf[i].x;
+            bim[buf][k * NTP + col] = pf[i].y;
         }
-        __syncthreads();
+    };
+    load(tile);
+    stage(0);
+    __syncthreads();
+    for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+        const bool more = tile + gridDim.x < ntiles;    // uniform over the block
+        if (more) load(tile + gridDim.x);
         Acc cr[NCB], ci[NCB], cs[GAUSS ? NCB : 1];
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
@@ -288,13 +310,15 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
             ci[c] = Acc{0, 0, 0, 0};
             if constexpr (GAUSS) cs[c] = Acc{0, 0, 0, 0};
         }
+        const R* __restrict__ pre = bre[buf];
+        const R* __restrict__ pim = bim[buf];
 #pragma unroll
         for (int q = 0; q < KS; q++) {
             const int kk = 4 * q + kq;
 #pragma unroll
             for (int c = 0; c < NCB; c++) {
-                R br = bre[kk * NTP + c * 16 + (lane & 15)];
-                R bi = bim[kk * NTP + c * 16 + (lane & 15)];
+                R br = pre[kk * NTP + c * 16 + (lane & 15)];
+                R bi = pim[kk * NTP + c * 16 + (lane & 15)];
                 if constexpr (GAUSS) {
                     cr[c] = mfma16x16x4(ar[q], br, cr[c]);        // T1
                     ci[c] = mfma16x16x4(ai[q], bi, ci[c]);        // T2
@@ -315,28 +339,31 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
                 ci[c] = cs[c] - t1 - t2;
             }
         }
-        __syncthreads();
-        // C tile back into the planes (as [row][col]) for a coalesced store
+        __syncthreads();                // every wave is done reading B from `buf`
+        // C tile into `buf` (as [row][col]); the next B into the other buffer
+        // (its last readers finished before the barrier above)
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 int row = wave * 16 + mfma_row<R>(lane, v);
                 int col = c * 16 + (lane & 15);
-                bre[row * NTP + col] = cr[c][v];
-                bim[row * NTP + col] = ci[c][v];
+                bre[buf][row * NTP + col] = cr[c][v];
+                bim[buf][row * NTP + col] = ci[c][v];
             }
         }
+        if (more) stage(buf ^ 1);
         __syncthreads();
         cplx<R>* __restrict__ dst = out + tile * NT * K;
-        for (int e = threadIdx.x; e < NT * K; e += nthreads) {
-            int col = e / K, k = e % K;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int e = threadIdx.x + i * NTH;
+            const int col = e / K, k = e % K;
             cplx<R> v;
-            v.x = bre[k * NTP + col];
-            v.y = bim[k * NTP + col];
+            v.x = bre[buf][k * NTP + col];
+            v.y = bim[buf][k * NTP + col];
             dst[e] = v;
         }
-        __syncthreads();
     }
 }
 
@@ -363,13 +390,13 @@ static hq_status launch_dense_t(hq_state s, const void* d_mat, int mq, void* in,
     const uint64_t K = (uint64_t)1 << mq;
     const R* ure = (const R*)d_mat;
     const R* uim = ure + K * K;
-    // MFMA path needs whole column tiles
-    if (mq == 4 && s->amps >= 16 * 64) return launch_mfma<R, 4, 64>(s, ure, uim, in, out);
-    if (mq == 5 && s->amps >= 32 * 64) return launch_mfma<R, 5, 64>(s, ure, uim, in, out);
-    // complex128 (three products, three accumulators per column block) takes
-    // 32-column tiles at K = 64 to stay within 256 registers (two waves per SIMD)
-    constexpr int nt6 = sizeof(R) == 8 ? 32 : 64;
-    if (mq == 6 && s->amps >= 64 * nt6) return launch_mfma<R, 6, nt6>(s, ure, uim, in, out);
+    // MFMA path needs whole column tiles.  complex128 (three products, three
+    // accumulators per column block, the next tile's B held in registers)
+    // takes 32-column tiles to stay within 256 registers (two waves per SIMD)
+    constexpr int nt = sizeof(R) == 8 ? 32 : 64;
+    if (mq == 4 && s->amps >= 16 * nt) return launch_mfma<R, 4, nt>(s, ure, uim, in, out);
+    if (mq == 5 && s->amps >= 32 * nt) return launch_mfma<R, 5, nt>(s, ure, uim, in, out);
+    if (mq == 6 && s->amps >= 64 * nt) return launch_mfma<R, 6, nt>(s, ure, uim, in, out);
     if (mq == 7 && s->amps >= 128 * 32) return launch_mfma<R, 7, 32>(s, ure, uim, in, out);
     uint64_t blocks = std::min<uint64_t>((s->amps + 255) / 256, 1u << 20);
     hipLaunchKernelGGL(k_dense_simple<R>, dim3((unsigned)blocks), dim3(256), 0, s->stream, (const cplx<R>*)in,

@@ -434,6 +434,11 @@ hq_status hq_batch_create(hq_engine e, const hq_circuit *circuits, int count, in
 hq_status hq_batch_run(hq_batch b, double *device_ms);
 hq_status hq_batch_results(hq_batch b, double *expect3, uint64_t *topk_idx, double *topk_prob);
 hq_status hq_batch_info(hq_batch b, int *count, int *rounds, int *launches, uint64_t *device_bytes);
+/* Algorithmic HBM bytes of one replay (no counterpart in the reference): every
+ * pass reads and writes its circuit's state once, except that the first pass
+ * only writes and later passes skip the tiles still known to be zero; divided
+ * by device_ms this is the batched pass's roofline figure. */
+hq_status hq_batch_pass_bytes(hq_batch b, uint64_t *bytes);
 hq_status hq_batch_destroy(hq_batch b);
 
 /* The k (<= 16) most probable physical basis states of s, probability

@@ -135,6 +135,25 @@ def test_dense_runs(gpu, mq):
     assert rel_err(s.download(), want) <= 1e-12
 
 
+@pytest.mark.parametrize("prec,mq", [("c128", 4), ("c128", 6), ("c128", 7), ("c64", 6), ("c64", 7)])
+def test_dense_runs_many_tiles_per_block(gpu, prec, mq):
+    """24 qubits: every block of the GEMM runs several column tiles, so the
+    pipelined path (next tile's B loaded during the current tile's MFMAs,
+    two LDS buffers) is exercised past its first tile."""
+    n = 24
+    K = 1 << mq
+    U = _rand_unitary(K, 200 + mq)
+    colmajor = U.T.reshape(-1)
+    psi0 = _rand_state(n, 300 + mq)
+    want = oracle.dense_runs(psi0, n, colmajor, mq)
+    c64 = prec == "c64"
+    s = gpu.State(n, gpu.HQ_C64 if c64 else gpu.HQ_C128, with_buffer=True)
+    s.upload(psi0.astype(np.complex64) if c64 else psi0)
+    s.apply_dense(colmajor, mq)
+    assert rel_err(s.download(), want) <= (2e-6 if c64 else 1e-12)
+    s.close()
+
+
 @pytest.mark.parametrize("mq", [4, 6, 7])
 def test_dense_runs_c64(gpu, mq):
     n = 14

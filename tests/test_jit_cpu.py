@@ -172,3 +172,27 @@ def rand_state_or_zero(n, seed, init):
     if init:
         return np.full(1 << n, 7.0 + 3.0j)
     return _rand_state(n, seed)
+
+
+RTC = os.path.join(os.path.dirname(HERE), "hyquas_amd", "hq_rtc")
+
+
+@pytest.mark.skipif(not os.path.exists(RTC), reason="compile helper not built")
+def test_compile_helper(hq, tmp_path):
+    """The compile helper (what the library runs, one child process per
+    concurrent compile) turns a generated group kernel into a gfx950 code
+    object without a GPU, and reports a bad source with the compiler log."""
+    n, mask = 14, (1 << 12) - 1
+    g = _group_for_mask(n, mask, 60, 3)
+    src, _, _ = hq.sharemem_source(n, g, mask, 1, 0, 4 | 8 | 16)
+    (tmp_path / "k.hip").write_text(src)
+    r = subprocess.run([RTC, str(tmp_path / "k.hip"), str(tmp_path / "k.co"), str(tmp_path / "k.log")], timeout=300)
+    assert r.returncode == 0
+    co = (tmp_path / "k.co").read_bytes()
+    assert co[:4] == b"\x7fELF" and int.from_bytes(co[18:20], "little") == 224   # EM_AMDGPU
+    assert b"hq_group_batch" in co
+    (tmp_path / "bad.hip").write_text(src.replace("__syncthreads();", "__syncthreads(;", 1))
+    r = subprocess.run([RTC, str(tmp_path / "bad.hip"), str(tmp_path / "bad.co"), str(tmp_path / "bad.log")],
+                       timeout=300)
+    assert r.returncode == 1 and not (tmp_path / "bad.co").exists()
+    assert "error" in (tmp_path / "bad.log").read_text()
