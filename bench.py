@@ -21,7 +21,9 @@ circuit on this host, scaled to the whole circuit.
 
 The default line also carries the metric's second half (--no-forms skips
 it): "batch", 4096 hardware-efficient-ansatz circuits batched in complex64
-against the FP64 hybrid path one circuit at a time on resident states and
+(specialised batched kernels, one per distinct group structure; their build
+time is reported, not timed) against the FP64 hybrid path one circuit at a
+time on resident states and
 the plain per-gate path (both on 128 evenly spaced circuits of the batch);
 "serve", a Poisson stream of 500 requests at each of --form-rates served by
 the complex64 engine against the FP64 hybrid path one circuit at a time
@@ -118,9 +120,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256,
                     help="batch: circuits of the plain FP64 one-at-a-time baseline (evenly spaced)")
     ap.add_argument("--no-autotune", action="store_true", help="batch: default tiles instead of autotuned ones")
-    ap.add_argument("--batch-jit", type=int, choices=[0, 1], default=0,
-                    help="batch: 1 specialised batched kernels shared by same-structure groups, "
-                         "0 the interpreter kernel with autotuned tiles (default)")
+    ap.add_argument("--batch-jit", type=int, choices=[0, 1], default=1,
+                    help="batch: 1 specialised batched kernels shared by same-structure groups (default), "
+                         "0 the interpreter kernel with autotuned tiles")
     ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
     ap.add_argument("--rates", default="20,100", help="serve: arrival rates (requests/s), comma separated")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain per-gate baseline")

@@ -37,8 +37,8 @@ struct CostModel {
                                  5.428e+12, 4.831e+12, 5.215e+12, 5.285e+12, 5.377e+12, 5.198e+12};
     double valu_rate = 3.793e+13;    // FP64 lane-ops/s achieved (complex128)
     double valu_rate_f32 = 3.84e+13;
-    double mfma_f64 = 3.886e+13;     // achieved FP64 MFMA flop/s (dense kernel, K = 64)
-    double mfma_f32 = 1.124e+14;     // achieved FP32 MFMA flop/s (dense kernel, K = 128)
+    double mfma_f64 = 7.730e+13;     // achieved FP64 MFMA flop/s (dense kernel, K = 64; algorithmic 8K per amplitude)
+    double mfma_f32 = 1.303e+14;     // achieved FP32 MFMA flop/s (dense kernel, K = 128)
     double lds_per_amp_stage = 0.0;      // s per amplitude per LDS exchange (0: from bandwidth)
     double launch_overhead = 2.642e-06;   // s per kernel launch
     double lds_bw = 256 * 128 * 2.4e9;   // B/s whole chip (128 B/clk/CU)
