@@ -109,6 +109,8 @@ def main():
                          "2 whole-line variant everywhere")
     ap.add_argument("--mode", choices=["mix", "group", "blas", "serial"], default="mix")
     ap.add_argument("--blas-mat", type=int, default=0, help="TransMM matrix qubits in blas mode (0: default)")
+    ap.add_argument("--transmm-inplace", type=int, choices=[-1, 0, 1], default=-1,
+                    help="TransMM groups: 1 one in-place dense pass, 0 transpose + GEMM (-1: library default, 1)")
     ap.add_argument("--jit", type=int, choices=[-1, 0, 1], default=-1,
                     help="specialised group kernels: -1 auto (from 22 qubits), 0 interpreter, 1 always")
     ap.add_argument("--cost-model", default="", help="parameter file for the partitioner's cost model")
@@ -169,6 +171,8 @@ def main():
         c.set_option("blas_mat", a.blas_mat)
     c.set_option("jit", a.jit)
     c.set_option("tune", a.tune)
+    if a.transmm_inplace >= 0:
+        c.set_option("transmm_inplace", a.transmm_inplace)
     if a.beam:
         c.set_option("beam", a.beam)
     if a.starts:
@@ -230,20 +234,25 @@ def main():
         kinds[k] = kinds.get(k, 0.0) + ms
     groups_pg = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_PERGATE)
     groups_blas = sum(1 for g in sched if g["backend"] == hq.HQ_BACKEND_BLAS)
-    # whole-step roofline: every group needs one HBM pass (TransMM two)
-    t_ideal = (sum(pass_units) + 2 * groups_blas) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
+    # whole-step roofline: every group needs one HBM pass (TransMM in the
+    # transpose form two)
+    dense = [ms for k, ms in launches if k == hq.KIND_DENSE]
+    trans = [ms for k, ms in launches if k == hq.KIND_TRANSPOSE]
+    blas_passes = (len(dense) + len(trans)) / a.steps
+    t_ideal = (sum(pass_units) + blas_passes) * bytes_per_pass / (HBM_PEAK_GBS * 1e9)
 
     # TransMM groups: GEMM flops (8 K 2^n per group) over the dense kernels'
     # time, transposes as bytes moved (read + write of the state)
     transmm = None
-    dense = [ms for k, ms in launches if k == hq.KIND_DENSE]
-    trans = [ms for k, ms in launches if k == hq.KIND_TRANSPOSE]
     if dense:
         flops = sum(8.0 * (1 << g["mat_qubit"]) * (1 << n) for g in sched
                     if g["backend"] == hq.HQ_BACKEND_BLAS) * a.steps
         mfma_peak = 78.6 if prec == hq.HQ_C128 else 157.3
         tflops = flops / (sum(dense) * 1e-3) / 1e12
-        transmm = {"dense_launches": len(dense), "dense_avg_ms": round(sum(dense) / len(dense), 3),
+        davg = sum(dense) / len(dense)
+        transmm = {"form": "transpose + GEMM" if trans else "in place",
+                   "dense_launches": len(dense), "dense_avg_ms": round(davg, 3),
+                   "dense_GBps": round(bytes_per_pass / (davg * 1e-3) / 1e9, 1),
                    "dense_TFLOPs": round(tflops, 2), "mfma_peak_TFLOPs": mfma_peak,
                    "mfma_frac": round(tflops / mfma_peak, 4),
                    "mat_qubits": sorted({g["mat_qubit"] for g in sched if g["backend"] == hq.HQ_BACKEND_BLAS})}
@@ -355,8 +364,8 @@ def _one_at_a_time(hq, specs, kind, mode):
     kernel build) on resident states, one per qubit count allocated up front;
     "plain" = per-gate kernels, a fresh state per circuit.  Returns (circuits/s,
     [sum Z_i Z_(i+1)])."""
-    from hyquas_amd.serving import build_circuit
-    states = ({n: hq.State(n, hq.HQ_C128, with_buffer=True) for n in sorted({s[0] for s in specs})}
+    from hyquas_amd.serving import INPLACE_MIN_QUBITS, build_circuit
+    states = ({n: hq.State(n, hq.HQ_C128, with_buffer=n < INPLACE_MIN_QUBITS) for n in sorted({s[0] for s in specs})}
               if mode == "hybrid" else {})
     out = []
     t0 = time.perf_counter()

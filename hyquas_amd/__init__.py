@@ -81,6 +81,7 @@ def _declare(L):
         "hq_apply_gates_plain": (S, [c_void_p, POINTER(HQGate), c_int]),
         "hq_program_create_sharemem": (S, [c_void_p, POINTER(HQGate), c_int, c_uint64, POINTER(c_void_p)]),
         "hq_program_create_blas": (S, [c_void_p, POINTER(c_int), POINTER(c_double), c_int, POINTER(c_void_p)]),
+        "hq_program_create_dense": (S, [c_void_p, POINTER(c_int), POINTER(c_double), c_int, POINTER(c_void_p)]),
         "hq_program_launch": (S, [c_void_p, c_void_p]),
         "hq_program_destroy": (S, [c_void_p]),
         "hq_program_info": (S, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
@@ -413,6 +414,16 @@ class State:
         else:
             m, mp = None, None
         _check(lib().hq_program_create_blas(self.handle, pp, mp, mat_qubits, byref(h)))
+        return Program(h)
+
+    def program_dense(self, positions, mat) -> Program:
+        """U (column-major, K = 2^len(positions)) on the qubits at `positions`
+        (matrix index bit j = positions[j]), in place (hq_program_create_dense)."""
+        h = c_void_p()
+        pp = (c_int * len(positions))(*positions)
+        m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex128).reshape(-1))
+        _check(lib().hq_program_create_dense(self.handle, pp, m.view(np.float64).ctypes.data_as(POINTER(c_double)),
+                                             len(positions), byref(h)))
         return Program(h)
 
     def launch(self, prog: Program):

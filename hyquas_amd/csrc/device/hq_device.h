@@ -144,7 +144,7 @@ struct ShmPlan {
 }  // namespace hq
 
 struct hq_program_s {
-    int kind = 0;            // 1 sharemem, 2 blas
+    int kind = 0;            // 1 sharemem, 2 blas (transpose + GEMM), 3 dense in place
     int n = 0;
     int precision = HQ_C128;
     // sharemem
@@ -158,6 +158,7 @@ struct hq_program_s {
     void* d_targs = nullptr; // transpose launch arguments (device)
     void* d_mat = nullptr;   // K x K, column-major, split planes re | im, in state precision
     int mat_qubits = 0;
+    uint64_t rows_mask = 0;  // dense in place: the qubits U acts on (row bit j = j-th lowest)
     // specialised (JIT) OShareMem kernel; null when the interpreter runs the group
     hipModule_t jit_module = nullptr;
     hipFunction_t jit_fn = nullptr;
@@ -171,6 +172,8 @@ hq_status launch_sharemem(hq_state s, hq_program p);
 hq_status transpose_prepare(hq_state s, const int* perm, void** dev_args);
 hq_status transpose_launch(hq_state s, const void* dev_args, void* in, void* out);
 hq_status launch_dense(hq_state s, const void* d_mat, int mat_qubits, void* in, void* out);
+// U (rows in the order of the set bits of `rows`) on those qubits, in place.
+hq_status launch_dense_at(hq_state s, const void* d_mat, int mat_qubits, uint64_t rows);
 int sharemem_default_rb(int precision, int tile);
 // io: bit 0 adds a leading, bit 1 a trailing stage without gates where the
 // first / last stage holds one of the three lowest tile bits in registers

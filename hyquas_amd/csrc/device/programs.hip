@@ -22,6 +22,91 @@ hq_status hq_program_create_sharemem(hq_state s, const hq_gate* gates, int num_g
     return build_sharemem_program(s, gates, num_gates, shared_mask, 0, out);
 }
 
+// U as split planes (re | im) in the state's precision: FP64 fusion, rounded
+// once to FP32 for complex64 states; uploaded to p->d_mat
+static hq_status upload_matrix(hq_state s, hq_program p, const double* mat, int mat_qubits, const char* who)
+{
+    const size_t K = (size_t)1 << mat_qubits;
+    const size_t rbytes = s->precision == HQ_C128 ? 8 : 4;
+    std::vector<char> planes(2 * K * K * rbytes);
+    for (size_t i = 0; i < K * K; i++) {
+        double re = mat[2 * i], im = mat[2 * i + 1];
+        if (!std::isfinite(re) || !std::isfinite(im)) {
+            set_error(std::string(who) + ": matrix entry is not finite");
+            return HQ_ERR_ARG;
+        }
+        if (rbytes == 8) {
+            ((double*)planes.data())[i] = re;
+            ((double*)planes.data())[K * K + i] = im;
+        } else {
+            ((float*)planes.data())[i] = (float)re;
+            ((float*)planes.data())[K * K + i] = (float)im;
+        }
+    }
+    hipError_t e = hipMalloc(&p->d_mat, planes.size());
+    if (e != hipSuccess) {
+        set_error(std::string(who) + ": hipMalloc: " + hipGetErrorString(e));
+        return HQ_ERR_OOM;
+    }
+    e = hipMemcpy(p->d_mat, planes.data(), planes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string(who) + ": hipMemcpy: " + hipGetErrorString(e));
+        return HQ_ERR_HIP;
+    }
+    return HQ_OK;
+}
+
+hq_status hq_program_create_dense(hq_state s, const int* positions, const double* mat, int mat_qubits,
+                                  hq_program* out)
+{
+    HQ_CHECK(s && out && positions && mat, HQ_ERR_ARG, "hq_program_create_dense: null argument");
+    *out = nullptr;
+    HQ_CHECK(mat_qubits >= 4 && mat_qubits <= 7, HQ_ERR_ARG, "hq_program_create_dense: mat_qubits must be 4..7");
+    HQ_CHECK(s->n >= dense_at_min_qubits(s->precision, mat_qubits), HQ_ERR_ARG,
+             "hq_program_create_dense: the state is smaller than one tile of the dense kernel");
+    const int m = mat_qubits;
+    uint64_t rows = 0;
+    for (int j = 0; j < m; j++) {
+        HQ_CHECK(positions[j] >= 0 && positions[j] < s->n && !((rows >> positions[j]) & 1), HQ_ERR_ARG,
+                 "hq_program_create_dense: positions must be distinct qubits of the state");
+        rows |= 1ull << positions[j];
+    }
+    // the kernel numbers the matrix rows by increasing qubit position: row
+    // index bit jj of the kernel is matrix bit sigma[jj]
+    int sigma[7];
+    for (int jj = 0, b = 0; b < 64 && jj < m; b++) {
+        if (!((rows >> b) & 1)) continue;
+        for (int j = 0; j < m; j++)
+            if (positions[j] == b) sigma[jj] = j;
+        jj++;
+    }
+    const size_t K = (size_t)1 << m;
+    std::vector<size_t> orig(K);
+    for (size_t x = 0; x < K; x++) {
+        size_t o = 0;
+        for (int jj = 0; jj < m; jj++)
+            if ((x >> jj) & 1) o |= (size_t)1 << sigma[jj];
+        orig[x] = o;
+    }
+    std::vector<double> pm(2 * K * K);
+    for (size_t c = 0; c < K; c++)
+        for (size_t r = 0; r < K; r++) {
+            const size_t from = orig[c] * K + orig[r], to = c * K + r;
+            pm[2 * to] = mat[2 * from];
+            pm[2 * to + 1] = mat[2 * from + 1];
+        }
+    hq_program p = new hq_program_s();
+    p->kind = 3;
+    p->n = s->n;
+    p->precision = s->precision;
+    p->mat_qubits = m;
+    p->rows_mask = rows;
+    hq_status st = upload_matrix(s, p, pm.data(), m, "hq_program_create_dense");
+    if (st != HQ_OK) { hq_program_destroy(p); return st; }
+    *out = p;
+    return HQ_OK;
+}
+
 hq_status hq_program_create_blas(hq_state s, const int* perm, const double* mat, int mat_qubits, hq_program* out)
 {
     HQ_CHECK(s && out, HQ_ERR_ARG, "hq_program_create_blas: null argument");
@@ -55,38 +140,8 @@ hq_status hq_program_create_blas(hq_state s, const int* perm, const double* mat,
         if (st != HQ_OK) { hq_program_destroy(p); return st; }
     }
     if (mat_qubits == 0) { *out = p; return HQ_OK; }
-    // U as split planes (re | im) in the state's precision: FP64 fusion,
-    // rounded once to FP32 for complex64 states
-    const size_t K = (size_t)1 << mat_qubits;
-    const size_t rbytes = s->precision == HQ_C128 ? 8 : 4;
-    std::vector<char> planes(2 * K * K * rbytes);
-    for (size_t i = 0; i < K * K; i++) {
-        double re = mat[2 * i], im = mat[2 * i + 1];
-        if (!std::isfinite(re) || !std::isfinite(im)) {
-            hq_program_destroy(p);
-            set_error("hq_program_create_blas: matrix entry is not finite");
-            return HQ_ERR_ARG;
-        }
-        if (rbytes == 8) {
-            ((double*)planes.data())[i] = re;
-            ((double*)planes.data())[K * K + i] = im;
-        } else {
-            ((float*)planes.data())[i] = (float)re;
-            ((float*)planes.data())[K * K + i] = (float)im;
-        }
-    }
-    hipError_t e = hipMalloc(&p->d_mat, planes.size());
-    if (e != hipSuccess) {
-        hq_program_destroy(p);
-        set_error(std::string("hq_program_create_blas: hipMalloc: ") + hipGetErrorString(e));
-        return HQ_ERR_OOM;
-    }
-    e = hipMemcpy(p->d_mat, planes.data(), planes.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        hq_program_destroy(p);
-        set_error(std::string("hq_program_create_blas: hipMemcpy: ") + hipGetErrorString(e));
-        return HQ_ERR_HIP;
-    }
+    st = upload_matrix(s, p, mat, mat_qubits, "hq_program_create_blas");
+    if (st != HQ_OK) { hq_program_destroy(p); return st; }
     *out = p;
     return HQ_OK;
 }
@@ -102,6 +157,13 @@ hq_status hq_program_launch(hq_state s, hq_program p)
         st = prof_begin(s, 1, &slot);
         if (st != HQ_OK) return st;
         st = p->jit_fn ? launch_sharemem_jit(s, p) : launch_sharemem(s, p);
+        if (st != HQ_OK) return st;
+        return prof_end(s, slot);
+    }
+    if (p->kind == 3) {
+        st = prof_begin(s, 3, &slot);
+        if (st != HQ_OK) return st;
+        st = launch_dense_at(s, p->d_mat, p->mat_qubits, p->rows_mask);
         if (st != HQ_OK) return st;
         return prof_end(s, slot);
     }

@@ -232,6 +232,33 @@ template <typename R> __device__ __forceinline__ int mfma_row(int lane, int v);
 template <> __device__ __forceinline__ int mfma_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
 template <> __device__ __forceinline__ int mfma_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
+// Scatter the low bits of x to the set bits of m, in order (x's bit j goes
+// to m's j-th lowest set bit), and the inverse gather.
+__device__ __forceinline__ uint64_t deposit(uint64_t x, uint64_t m)
+{
+    uint64_t r = 0;
+    for (uint64_t b = 1; m; b <<= 1, m &= m - 1)
+        if (x & b) r |= m & (~m + 1);
+    return r;
+}
+__device__ __forceinline__ uint64_t extract(uint64_t x, uint64_t m)
+{
+    uint64_t r = 0;
+    for (uint64_t b = 1; m; b <<= 1, m &= m - 1)
+        if (x & m & (~m + 1)) r |= b;
+    return r;
+}
+
+// Which amplitudes a GEMM tile holds: the K matrix rows are the amplitude
+// bits `rows` (row index bit j = the j-th lowest bit of `rows`), the NT tile
+// columns the bits `cols`, the tile index the remaining bits `blocks`.  With
+// rows = the MQ lowest bits and cols the next ones this is the contiguous
+// layout a transpose leaves; any other placement applies U to those qubits
+// where they are, in one pass.
+struct DenseMasks {
+    uint64_t rows, cols, blocks;
+};
+
 // One wave per 16-row block of U (K/16 waves), NT columns per tile.
 //
 // GAUSS: the complex product with three real MFMAs instead of four (Gauss's
@@ -246,11 +273,19 @@ template <> __device__ __forceinline__ int mfma_row<double>(int lane, int v) { r
 // into the other buffer afterwards, so the load's latency hides behind the
 // MFMAs; the C tile goes back through the current buffer for coalesced
 // stores.  Two barriers per tile.
+//
+// Addressing: element t of a tile (t over the bits rows|cols in increasing
+// order, so consecutive t are consecutive amplitudes as far as those bits
+// are contiguous) sits at deposit(tile, blocks) + deposit(t, rows|cols);
+// thread x handles t = x + i * NTH, whose offset splits into a per-thread
+// part (deposit of x) and a uniform part (deposit of i * NTH), and likewise
+// its LDS slot.  `in` may equal `out`: a block reads a whole tile before it
+// writes it, and tiles are disjoint.
 template <typename R, int MQ, int NT, bool GAUSS>
-__global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx<R>* __restrict__ in,
-                                                                     cplx<R>* __restrict__ out,
+__global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx<R>* in, cplx<R>* out,
                                                                      const R* __restrict__ ure,
-                                                                     const R* __restrict__ uim, uint64_t ncols)
+                                                                     const R* __restrict__ uim, DenseMasks dm,
+                                                                     uint64_t ntiles)
 {
     constexpr int K = 1 << MQ;
     constexpr int KS = K / 4;          // k-steps
@@ -268,41 +303,77 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
     const int arow = wave * 16 + (lane & 15);
     const int kq = lane >> 4;          // 0..3
 
-    // U fragments of this wave's row block: A[row][4q + kq]
-    R ar[KS], ai[KS], as[GAUSS ? KS : 1];
+    // U fragments of this wave's row block: A[row][4q + kq]; at K = 128 the
+    // sums Ar + Ai are formed per use (registers are full)
+    constexpr bool KEEP_AS = GAUSS;
+    R ar[KS], ai[KS], as[KEEP_AS ? KS : 1];
 #pragma unroll
     for (int q = 0; q < KS; q++) {
         int kk = 4 * q + kq;
         ar[q] = ure[kk * K + arow];     // column-major: U(row, col) at col*K + row
         ai[q] = uim[kk * K + arow];
-        if constexpr (GAUSS) as[q] = ar[q] + ai[q];
+        if constexpr (KEEP_AS) as[q] = ar[q] + ai[q];
     }
 
-    const uint64_t ntiles = ncols / NT;
     uint64_t tile = blockIdx.x;
     if (tile >= ntiles) return;         // uniform: the grid never exceeds the tiles
-    cplx<R> pf[PER];
-    auto load = [&](uint64_t t) {
-        const cplx<R>* __restrict__ src = in + t * NT * K;
+    const uint64_t tm = dm.rows | dm.cols;
+    const uint64_t aoff = deposit(threadIdx.x, tm);
+    const int alds = (int)extract(aoff, dm.rows) * NTP + (int)extract(aoff, dm.cols);
+    // i * NTH sets only the top log2(PER) tile-index bits: element i's
+    // uniform offset is a sum over i's bits of one amplitude offset and one
+    // LDS offset per bit (kept in scalar registers)
+    constexpr int PB = __builtin_ctz((unsigned)PER);
+    uint64_t hoff[PB > 0 ? PB : 1];
+    int hlds[PB > 0 ? PB : 1];
 #pragma unroll
-        for (int i = 0; i < PER; i++) pf[i] = src[threadIdx.x + i * NTH];
+    for (int j = 0; j < PB; j++) {
+        hoff[j] = deposit((uint64_t)NTH << j, tm);
+        hlds[j] = (int)extract(hoff[j], dm.rows) * NTP + (int)extract(hoff[j], dm.cols);
+    }
+    auto boff = [&](int i) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < PB; j++)
+            if ((i >> j) & 1) o += hoff[j];
+        return o;
+    };
+    auto blds = [&](int i) {
+        int o = 0;
+#pragma unroll
+        for (int j = 0; j < PB; j++)
+            if ((i >> j) & 1) o += hlds[j];
+        return o;
+    };
+    // tile bases: this block's tiles are blockIdx.x + k * gridDim.x (gridDim a
+    // power of two), so the base is deposit(blockIdx.x, blocks) plus the k-th
+    // subset of the block bits above the lowest log2(gridDim) ones, stepped
+    // with ((cur | ~hi) + 1) & hi
+    uint64_t hi = dm.blocks;
+    for (unsigned g = gridDim.x; g > 1; g >>= 1) hi &= hi - 1;
+    const uint64_t base0 = deposit(tile, dm.blocks);
+    auto next_sub = [hi](uint64_t cur) { return ((cur | ~hi) + 1) & hi; };
+    cplx<R> pf[PER];
+    auto load = [&](uint64_t base) {
+        const cplx<R>* src = in + base + aoff;
+#pragma unroll
+        for (int i = 0; i < PER; i++) pf[i] = src[boff(i)];
     };
     // B (K x NT) as planes [k][col]
     auto stage = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int e = threadIdx.x + i * NTH;
-            const int col = e / K, k = e % K;
-            bre[buf][k * NTP + col] = pf[i].x;
-            bim[buf][k * NTP + col] = pf[i].y;
+            bre[buf][alds + blds(i)] = pf[i].x;
+            bim[buf][alds + blds(i)] = pf[i].y;
         }
     };
-    load(tile);
+    uint64_t sub = 0;                   // the current tile's subset of hi
+    load(base0);
     stage(0);
     __syncthreads();
-    for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+    for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1, sub = next_sub(sub)) {
         const bool more = tile + gridDim.x < ntiles;    // uniform over the block
-        if (more) load(tile + gridDim.x);
+        if (more) load(base0 + next_sub(sub));
         Acc cr[NCB], ci[NCB], cs[GAUSS ? NCB : 1];
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
@@ -320,9 +391,12 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
                 R br = pre[kk * NTP + c * 16 + (lane & 15)];
                 R bi = pim[kk * NTP + c * 16 + (lane & 15)];
                 if constexpr (GAUSS) {
+                    R asq;
+                    if constexpr (KEEP_AS) asq = as[q];
+                    else asq = ar[q] + ai[q];
                     cr[c] = mfma16x16x4(ar[q], br, cr[c]);        // T1
                     ci[c] = mfma16x16x4(ai[q], bi, ci[c]);        // T2
-                    cs[c] = mfma16x16x4(as[q], br + bi, cs[c]);   // T3
+                    cs[c] = mfma16x16x4(asq, br + bi, cs[c]);     // T3
                 } else {
                     cr[c] = mfma16x16x4(ar[q], br, cr[c]);
                     cr[c] = mfma16x16x4(ai[q], -bi, cr[c]);
@@ -354,15 +428,13 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
         }
         if (more) stage(buf ^ 1);
         __syncthreads();
-        cplx<R>* __restrict__ dst = out + tile * NT * K;
+        cplx<R>* dst = out + base0 + sub + aoff;
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int e = threadIdx.x + i * NTH;
-            const int col = e / K, k = e % K;
             cplx<R> v;
-            v.x = bre[buf][k * NTP + col];
-            v.y = bim[buf][k * NTP + col];
-            dst[e] = v;
+            v.x = bre[buf][alds + blds(i)];
+            v.y = bim[buf][alds + blds(i)];
+            dst[boff(i)] = v;
         }
     }
 }
@@ -371,18 +443,37 @@ __global__ __launch_bounds__(64 * ((1 << MQ) / 16)) void k_dense_mfma(const cplx
 // limit at K >= 64); complex64 keeps four products (FP32 matrix cores run at
 // twice the FP64 rate and Gauss's form adds rounding at single precision)
 template <typename R, int MQ, int NT>
-static hq_status launch_mfma(hq_state s, const R* ure, const R* uim, void* in, void* out)
+static hq_status launch_mfma(hq_state s, const R* ure, const R* uim, const void* in, void* out, DenseMasks dm)
 {
     constexpr bool gauss = sizeof(R) == 8;
     constexpr int K = 1 << MQ;
-    uint64_t ncols = s->amps / K;
-    uint64_t ntiles = ncols / NT;
-    uint64_t grid = std::min<uint64_t>(ntiles, 2048);
+    const uint64_t ntiles = s->amps / ((uint64_t)K * NT);
+    const uint64_t grid = std::min<uint64_t>(ntiles, 2048);
     hipLaunchKernelGGL((k_dense_mfma<R, MQ, NT, gauss>), dim3((unsigned)grid), dim3(64 * (K / 16)), 0, s->stream,
-                       (const cplx<R>*)in, (cplx<R>*)out, ure, uim, ncols);
+                       (const cplx<R>*)in, (cplx<R>*)out, ure, uim, dm, ntiles);
     HQ_HIP(hipGetLastError());
     return HQ_OK;
 }
+
+// Column tile width: complex128 (three products, three accumulators per
+// column block, the next tile's B held in registers) takes 32-column tiles to
+// stay within 256 registers (two waves per SIMD)
+template <typename R> constexpr int dense_nt(int mq) { return sizeof(R) == 8 || mq == 7 ? 32 : 64; }
+
+template <typename R>
+static hq_status launch_mfma_masks(hq_state s, const R* ure, const R* uim, int mq, const void* in, void* out,
+                                   DenseMasks dm)
+{
+    switch (mq) {
+    case 4: return launch_mfma<R, 4, dense_nt<R>(4)>(s, ure, uim, in, out, dm);
+    case 5: return launch_mfma<R, 5, dense_nt<R>(5)>(s, ure, uim, in, out, dm);
+    case 6: return launch_mfma<R, 6, dense_nt<R>(6)>(s, ure, uim, in, out, dm);
+    case 7: return launch_mfma<R, 7, dense_nt<R>(7)>(s, ure, uim, in, out, dm);
+    default: set_error("dense kernel: matrix qubits must be 4..7"); return HQ_ERR_UNSUPPORTED;
+    }
+}
+
+static uint64_t all_bits(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1); }
 
 template <typename R>
 static hq_status launch_dense_t(hq_state s, const void* d_mat, int mq, void* in, void* out)
@@ -390,14 +481,15 @@ static hq_status launch_dense_t(hq_state s, const void* d_mat, int mq, void* in,
     const uint64_t K = (uint64_t)1 << mq;
     const R* ure = (const R*)d_mat;
     const R* uim = ure + K * K;
-    // MFMA path needs whole column tiles.  complex128 (three products, three
-    // accumulators per column block, the next tile's B held in registers)
-    // takes 32-column tiles to stay within 256 registers (two waves per SIMD)
-    constexpr int nt = sizeof(R) == 8 ? 32 : 64;
-    if (mq == 4 && s->amps >= 16 * nt) return launch_mfma<R, 4, nt>(s, ure, uim, in, out);
-    if (mq == 5 && s->amps >= 32 * nt) return launch_mfma<R, 5, nt>(s, ure, uim, in, out);
-    if (mq == 6 && s->amps >= 64 * nt) return launch_mfma<R, 6, nt>(s, ure, uim, in, out);
-    if (mq == 7 && s->amps >= 128 * 32) return launch_mfma<R, 7, 32>(s, ure, uim, in, out);
+    // MFMA path: the matrix qubits are the lowest, the tile columns the next
+    const int nt = mq >= 4 && mq <= 7 ? dense_nt<R>(mq) : 0;
+    if (nt && s->amps >= K * nt) {
+        DenseMasks dm;
+        dm.rows = K - 1;
+        dm.cols = (uint64_t)(nt - 1) << mq;
+        dm.blocks = all_bits(s->n) & ~(dm.rows | dm.cols);
+        return launch_mfma_masks<R>(s, ure, uim, mq, in, out, dm);
+    }
     uint64_t blocks = std::min<uint64_t>((s->amps + 255) / 256, 1u << 20);
     hipLaunchKernelGGL(k_dense_simple<R>, dim3((unsigned)blocks), dim3(256), 0, s->stream, (const cplx<R>*)in,
                        (cplx<R>*)out, ure, uim, mq, s->amps);
@@ -409,6 +501,40 @@ hq_status launch_dense(hq_state s, const void* d_mat, int mq, void* in, void* ou
 {
     return s->precision == HQ_C128 ? launch_dense_t<double>(s, d_mat, mq, in, out)
                                    : launch_dense_t<float>(s, d_mat, mq, in, out);
+}
+
+int dense_at_min_qubits(int precision, int mq)
+{
+    if (mq < 4 || mq > 7) return 1 << 20;
+    const int nt = precision == HQ_C128 ? dense_nt<double>(mq) : dense_nt<float>(mq);
+    return mq + __builtin_ctz((unsigned)nt);
+}
+
+// U on the qubits at the set bits of `rows` (row index bit j = the j-th
+// lowest of them), in place: tile columns along the lowest other bits.
+template <typename R>
+static hq_status launch_dense_at_t(hq_state s, const void* d_mat, int mq, uint64_t rows)
+{
+    const uint64_t K = (uint64_t)1 << mq;
+    const R* ure = (const R*)d_mat;
+    const R* uim = ure + K * K;
+    const int nt = dense_nt<R>(mq);
+    DenseMasks dm;
+    dm.rows = rows;
+    dm.cols = 0;
+    for (int b = 0, got = 0; b < s->n && got < __builtin_ctz((unsigned)nt); b++)
+        if (!((rows >> b) & 1)) { dm.cols |= 1ull << b; got++; }
+    dm.blocks = all_bits(s->n) & ~(dm.rows | dm.cols);
+    return launch_mfma_masks<R>(s, ure, uim, mq, s->data, s->data, dm);
+}
+
+hq_status launch_dense_at(hq_state s, const void* d_mat, int mq, uint64_t rows)
+{
+    HQ_CHECK(__builtin_popcountll((unsigned long long)rows) == mq && (rows & ~all_bits(s->n)) == 0 &&
+                 s->n >= dense_at_min_qubits(s->precision, mq),
+             HQ_ERR_ARG, "dense kernel: rows must name mat_qubits qubits of a large enough state");
+    return s->precision == HQ_C128 ? launch_dense_at_t<double>(s, d_mat, mq, rows)
+                                   : launch_dense_at_t<float>(s, d_mat, mq, rows);
 }
 
 }  // namespace hq

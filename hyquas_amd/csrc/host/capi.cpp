@@ -138,6 +138,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
             C.params.blasMat = 6;
             C.params.matLo = 4;
             C.params.matHi = 8;
+            C.params.blasInPlace = false;   // the reference's transposes and layouts
         } else {
             C.params.cost = CostModel::mi355x();
         }
@@ -181,6 +182,9 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
     } else if (k == "tune") {
         HQ_CARG(v >= -1 && v <= 2, "tune must be -1 (auto), 0, 1 or 2");
         C.exec.tune = (int)v;
+    } else if (k == "transmm_inplace") {
+        HQ_CARG(v == 0 || v == 1, "transmm_inplace must be 0 or 1");
+        C.params.blasInPlace = v == 1;
     } else if (k == "profile_events") {
         C.profileEvents = v != 0;
     } else {

@@ -68,10 +68,8 @@ hq_status Circuit::useState(hq_state s)
 
 bool Circuit::needsBuffer() const
 {
-    for (auto& lg : schedule.localGroups)
-        for (auto& gg : lg.fullGroups)
-            if (gg.backend == Backend::BLAS) return true;
-    return false;
+    // only TransMM groups that run out of place write through the work buffer
+    return Executor::needsBuffer(schedule, numQubits, params.precision);
 }
 
 hq_status Circuit::ensureState()
@@ -86,7 +84,8 @@ hq_status Circuit::ensureState()
             return HQ_ERR_ARG;
         }
         if (buf && !stateHasBuffer) {
-            hq::set_error("the schedule has TransMM groups: the state given with use_state needs a work buffer");
+            hq::set_error("the schedule has TransMM groups in transpose form: the state given with use_state needs a "
+                          "work buffer");
             return HQ_ERR_ARG;
         }
         return HQ_OK;
@@ -359,7 +358,7 @@ void Circuit::dumpGates() const
 qindex Circuit::toPhysicalID(qindex idx) const
 {
     qindex id = 0;
-    const auto& pos = schedule.finalState.pos;
+    const auto& pos = schedule.stateLayout().pos;
     for (int i = 0; i < numQubits; i++)
         if ((idx >> i) & 1) id |= qindex(1) << pos[i];
     return id;
@@ -368,7 +367,7 @@ qindex Circuit::toPhysicalID(qindex idx) const
 qindex Circuit::toLogicID(qindex idx) const
 {
     qindex id = 0;
-    const auto& pos = schedule.finalState.pos;
+    const auto& pos = schedule.stateLayout().pos;
     for (int i = 0; i < numQubits; i++)
         if ((idx >> pos[i]) & 1) id |= qindex(1) << i;
     return id;

@@ -74,7 +74,8 @@ public:
     // Run on a caller-owned state (borrowed: never destroyed here); nullptr
     // returns to states of the circuit's own.
     hq_status useState(hq_state s);
-    const std::vector<int>& finalPos() const { return schedule.finalState.pos; }
+    // logical -> physical position of each qubit in the state a run leaves
+    const std::vector<int>& finalPos() const { return schedule.stateLayout().pos; }
     double lastTimes[4] = {0, 0, 0, 0};   // ms by launch kind (profile on)
     std::vector<std::pair<int, double>> lastLaunches;
     bool profileEvents = false;

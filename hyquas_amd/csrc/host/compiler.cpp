@@ -9,6 +9,8 @@
 
 #include "compiler.h"
 
+#include "hq_internal.h"
+
 #include <algorithm>
 #include <cstring>
 
@@ -212,6 +214,12 @@ std::vector<int> GroupFinder::find(std::vector<char>& full, std::vector<qindex>&
 // Physical tile of an OShareMem group: its related (logical) qubits at their
 // physical positions, topped up with the lowest physical positions
 // (fillRelatedQubits, src/executor.cpp:585-596).
+// Whether a TransMM group of `ms` matrix qubits runs in place (no transpose pass).
+static bool blasInPlace(const CompilerParams& p, int n, int ms)
+{
+    return p.blasInPlace && n >= hq::dense_at_min_qubits(p.precision, std::max(ms, p.minMat));
+}
+
 static qindex tileMask(qindex relatedLogical, const State& st, int size)
 {
     qindex m = 0;
@@ -267,7 +275,7 @@ static Schedule greedySchedule(int n, const std::vector<Gate>& gates, const Comp
                 std::fill(related.begin(), related.end(), 0);
                 std::vector<int> idx = finder.find(full, related, false, ms, all);
                 if (idx.empty()) continue;
-                double eff = p.cost.perfBLAS(n, p.precision, ms) / idx.size();
+                double eff = p.cost.perfBLAS(n, p.precision, ms, blasInPlace(p, n, ms)) / idx.size();
                 if (eff < bestEff) {
                     ggIdx = idx;
                     backend = Backend::BLAS;
@@ -292,9 +300,9 @@ static Schedule greedySchedule(int n, const std::vector<Gate>& gates, const Comp
             for (int x : ggIdx) gs.push_back(&gates[x]);
             *cost += backend == Backend::PerGate
                          ? p.cost.perfPerGate(n, p.precision, gs, tileMask(gg.relatedQubits, state, localSize))
-                         : p.cost.perfBLAS(n, p.precision, blasMs);
+                         : p.cost.perfBLAS(n, p.precision, blasMs, blasInPlace(p, n, blasMs));
         }
-        state = gg.initState(state, n, p.minMat);
+        state = gg.initState(state, n, p.minMat, p.blasInPlace ? p.precision : -1);
         finder.remove(ggIdx);
         lg.relatedQubits |= gg.relatedQubits;
         lg.fullGroups.push_back(std::move(gg));

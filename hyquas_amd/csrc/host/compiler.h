@@ -18,6 +18,10 @@ struct CompilerParams {
     int window = 512;         // gates considered per group (OneLayerCompiler<512>)
     int starts = 4;           // OShareMem merge lookahead: 1 = reference rule
     int beam = 8;             // > 1: beam search over OShareMem-only partitions (kept if better)
+    // TransMM groups the in-place dense kernel can run keep the layout (no
+    // transpose planned, OShareMem groups after them coalesce on the actual
+    // low qubits); false: the reference's transposes (src/schedule.cpp:339-382)
+    bool blasInPlace = true;
     CostModel cost;
 };
 

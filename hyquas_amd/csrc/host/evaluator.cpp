@@ -96,14 +96,16 @@ double CostModel::perfPerGate(int n, int precision, const std::vector<const Gate
     return t * 1e3;
 }
 
-double CostModel::perfBLAS(int n, int precision, int matSize) const
+double CostModel::perfBLAS(int n, int precision, int matSize, bool inPlace) const
 {
     if (reference) return blas_perf[matSize] + cutt_cost;
     const double amps = std::ldexp(1.0, n);
     const double bytes = 2 * amps * (precision == HQ_C128 ? 16 : 8);
     const double flops = 8.0 * std::ldexp(1.0, matSize) * amps;
     const double mf = precision == HQ_C128 ? mfma_f64 : mfma_f32;
-    double t = bytes / hbm_bw + std::max(bytes / hbm_bw, flops / mf) + 2 * launch_overhead;
+    // transpose pass (unless in place) + the GEMM pass
+    double t = (inPlace ? 0.0 : bytes / hbm_bw + launch_overhead) + std::max(bytes / hbm_bw, flops / mf) +
+               launch_overhead;
     return t * 1e3;
 }
 

@@ -60,5 +60,6 @@ struct CostModel {
     // OShareMem pass bandwidth for a tile with run length L
     double shmBandwidth(int precision, int runLength) const;
     static int runLength(qindex tileMask);
-    double perfBLAS(int numQubits, int precision, int matSize) const;
+    // inPlace: the group's matrix runs where its qubits are, one pass, no transpose
+    double perfBLAS(int numQubits, int precision, int matSize, bool inPlace = false) const;
 };

@@ -5,7 +5,9 @@
 // physical positions through the current layout (src/executor.cpp:462-493,
 // 585-632); the device layer then splits the tile into register/thread bits
 // and plans the register stages.  BLAS (TransMM) groups: FP64 fusion on the
-// host (src/schedule.cpp:578-702) and a transpose + dense program.
+// host (src/schedule.cpp:578-702) and a transpose + dense program, or one
+// in-place dense pass over the group's qubits where they are (the layout
+// then stays; Schedule::runState records the layout a run leaves).
 
 #include "executor.h"
 
@@ -24,6 +26,25 @@ qindex Executor::fillRelated(qindex related, const State& st, int size)
     return related;
 }
 
+// A TransMM group runs in place when its plan kept the layout and the dense
+// kernel takes its matrix on this state.
+static bool runsInPlace(const GateGroup& gg, const State& st, int n, int precision)
+{
+    return gg.state.pos == st.pos && n >= hq::dense_at_min_qubits(precision, gg.matQubit);
+}
+
+bool Executor::needsBuffer(const Schedule& sch, int n, int precision)
+{
+    State st(n);
+    for (auto& lg : sch.localGroups)
+        for (auto& gg : lg.fullGroups) {
+            if (gg.backend != Backend::BLAS) continue;
+            if (!runsInPlace(gg, st, n, precision)) return true;
+            st = gg.state;
+        }
+    return false;
+}
+
 // Tiles smaller than 7 qubits (states below 7 qubits) run the group's gates
 // through the plain kernels: the register kernel needs 64 lanes x 2 registers.
 static const int kMinTile = 7;
@@ -31,6 +52,9 @@ static const int kMinTile = 7;
 hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& p)
 {
     State st(n);
+    int nq = n, precision = HQ_C128;
+    hq_status irc = hq_state_info(s, &nq, &precision);
+    if (irc != HQ_OK) return irc;
     std::vector<hq::SharememRequest> reqs;
     std::vector<GateGroup*> owners;
     bool first = true;
@@ -39,8 +63,9 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
     // qubit (diagonal gates and controls leave a |0> qubit at |0>), every
     // amplitude with that qubit set is zero, so tiles whose block address
     // sets such a qubit are zero before the group and stay zero.  `support`
-    // holds the physical qubits some earlier non-diagonal gate targeted; it
-    // is tracked until the first TransMM group (its permutation moves qubits).
+    // holds the physical qubits some earlier non-diagonal gate (or an
+    // in-place TransMM group) touched; it is tracked until the first TransMM
+    // group in the transpose form (its permutation moves qubits).
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     uint64_t support = 0;
     bool track = true;
@@ -89,18 +114,40 @@ hq_status Executor::prepare(Schedule& sch, int n, hq_state s, const ExecParams& 
                     track = false;
                 }
             } else if (gg.backend == Backend::BLAS) {
-                track = false;
                 gg.liveFraction = 1.0;
                 gg.initCPUMatrix();
                 const size_t K = (size_t)1 << gg.matQubit;
                 std::vector<double> m(2 * K * K);
                 for (size_t i = 0; i < K * K; i++) { m[2 * i] = gg.matrix[i].real(); m[2 * i + 1] = gg.matrix[i].imag(); }
-                hq_status rc = hq_program_create_blas(s, gg.cuttPerm.data(), m.data(), gg.matQubit, &gg.program);
-                if (rc != HQ_OK) return rc;
-                st = gg.state;
+                const std::vector<int> ml = gg.matLogical();
+                bool low = true;   // matrix bit j already at position j
+                for (int j = 0; j < gg.matQubit; j++) low = low && st.pos[ml[j]] == j;
+                if (runsInPlace(gg, st, n, precision)) {
+                    // one pass where the qubits are; the layout stays
+                    std::vector<int> positions(gg.matQubit);
+                    for (int j = 0; j < gg.matQubit; j++) {
+                        positions[j] = st.pos[ml[j]];
+                        support |= 1ull << positions[j];   // U mixes amplitudes across its qubits
+                    }
+                    hq_status rc = hq_program_create_dense(s, positions.data(), m.data(), gg.matQubit, &gg.program);
+                    if (rc != HQ_OK) return rc;
+                } else if (gg.state.pos == st.pos && !low) {
+                    hq::set_error("executor: a TransMM group planned in place is too large for the dense kernel on "
+                                  "this state (schedule loaded for another precision?)");
+                    return HQ_ERR_UNSUPPORTED;
+                } else {
+                    // the reference's transpose (identity when the qubits are
+                    // already lowest) + GEMM on runs of K amplitudes
+                    hq_status rc =
+                        hq_program_create_blas(s, gg.cuttPerm.data(), m.data(), gg.matQubit, &gg.program);
+                    if (rc != HQ_OK) return rc;
+                    st = gg.state;
+                    track = false;   // the permutation moves the qubits the tracking is about
+                }
             }
         }
     }
+    sch.runState = st;
     const bool jit = p.jit == 1 || (p.jit < 0 && n >= p.jitMinQubits);
     std::vector<hq_program> progs;
     const int tune = !jit ? 0 : p.tune >= 0 ? p.tune : (n >= p.tuneMinQubits ? 1 : 0);

@@ -28,4 +28,8 @@ public:
     // fillRelatedQubits (src/executor.cpp:585-596): top the group's qubits up
     // to `size` with the lowest physical positions.
     static qindex fillRelated(qindex related, const State& st, int size);
+    // Whether a run of sch needs the state's work buffer: a TransMM group
+    // whose plan moves qubits (the reference's transpose) or whose matrix the
+    // in-place kernel cannot take runs out of place.
+    static bool needsBuffer(const Schedule& sch, int numQubits, int precision);
 };

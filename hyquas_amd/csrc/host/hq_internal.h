@@ -29,6 +29,10 @@ struct SharememRequest {
     bool init = false;   // first group of a run: build |0...0> instead of loading the state
     uint64_t zero_mask = 0;  // physical qubits still at |0> before this group (their tiles are skipped)
 };
+// Smallest state for hq_program_create_dense with `mat_qubits` matrix qubits
+// (one whole tile of the in-place GEMM); mat_qubits outside 4..7: never.
+int dense_at_min_qubits(int precision, int mat_qubits);
+
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
                                   std::vector<hq_program>* out, int tune_mode = 0);
 }  // namespace hq

@@ -49,6 +49,7 @@ int main(int argc, char** argv)
     if (reference) {
         c->referenceProfile = true;
         c->params.localSize = 10;
+        c->params.blasInPlace = false;
     }
     try {
         c->compile();

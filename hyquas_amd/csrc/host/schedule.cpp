@@ -2,6 +2,8 @@
 
 #include "schedule.h"
 
+#include "hq_internal.h"
+
 #include <algorithm>
 #include <cstdio>
 #include <utility>
@@ -75,8 +77,19 @@ State GateGroup::initPerGateState(const State& old)
     return state;
 }
 
-State GateGroup::initBlasState(const State& old, int numLocalQubits, int minMatSize)
+State GateGroup::initBlasState(const State& old, int numLocalQubits, int minMatSize, int inPlacePrecision)
 {
+    if (inPlacePrecision >= 0) {
+        const int nm = __builtin_popcountll((unsigned long long)relatedQubits);
+        const int mq = std::min(std::max(nm, minMatSize), numLocalQubits);
+        if (numLocalQubits >= hq::dense_at_min_qubits(inPlacePrecision, mq)) {
+            matQubit = mq;
+            cuttPerm.resize(numLocalQubits);
+            for (int i = 0; i < numLocalQubits; i++) cuttPerm[i] = i;
+            state = old;
+            return state;
+        }
+    }
     // src/schedule.cpp:339-382: swap the group's qubits that sit above the
     // first |group| positions with the non-group qubits below it, pairwise in
     // position order; cuttPerm[newPos] = oldPos
@@ -105,10 +118,26 @@ State GateGroup::initBlasState(const State& old, int numLocalQubits, int minMatS
     return state;
 }
 
-State GateGroup::initState(const State& old, int numLocalQubits, int minMatSize)
+State GateGroup::initState(const State& old, int numLocalQubits, int minMatSize, int inPlacePrecision)
 {
-    if (backend == Backend::BLAS) return initBlasState(old, numLocalQubits, minMatSize);
+    if (backend == Backend::BLAS) return initBlasState(old, numLocalQubits, minMatSize, inPlacePrecision);
     return initPerGateState(old);
+}
+
+std::vector<int> GateGroup::matLogical() const
+{
+    const int n = (int)state.pos.size();
+    std::vector<std::pair<int, int>> sel;   // (position, logical qubit)
+    for (int q = 0; q < n; q++)
+        if ((relatedQubits >> q) & 1) sel.push_back({state.pos[q], q});
+    for (int p = 0; p < n && (int)sel.size() < matQubit; p++) {
+        const int q = state.layout[p];
+        if (!((relatedQubits >> q) & 1)) sel.push_back({p, q});
+    }
+    std::sort(sel.begin(), sel.end());
+    std::vector<int> out;
+    for (auto& x : sel) out.push_back(x.second);
+    return out;
 }
 
 void GateGroup::initCPUMatrix()
@@ -123,11 +152,16 @@ void GateGroup::initCPUMatrix()
         std::complex<double> m00, m01, m10, m11;
     };
     std::vector<Op> ops;
+    // matrix bit of each logical qubit (the position after the reference's
+    // transpose; the rank among the group's qubits when it runs in place)
+    const std::vector<int> ml = matLogical();
+    std::vector<int> bit(state.pos.size(), -1);
+    for (size_t j = 0; j < ml.size(); j++) bit[ml[j]] = (int)j;
     for (auto& g : gates) {
         Op o;
-        o.t = state.pos[g.targetQubit];
-        o.c1 = g.controlQubit >= 0 ? state.pos[g.controlQubit] : -1;
-        o.c2 = g.controlQubit2 >= 0 ? state.pos[g.controlQubit2] : -1;
+        o.t = bit[g.targetQubit];
+        o.c1 = g.controlQubit >= 0 ? bit[g.controlQubit] : -1;
+        o.c2 = g.controlQubit2 >= 0 ? bit[g.controlQubit2] : -1;
         o.m00 = {g.mat[0][0].x, g.mat[0][0].y};
         o.m01 = {g.mat[0][1].x, g.mat[0][1].y};
         o.m10 = {g.mat[1][0].x, g.mat[1][0].y};
@@ -157,6 +191,7 @@ void Schedule::clear()
 {
     localGroups.clear();
     finalState = State();
+    runState = State();
     initFused = false;
 }
 

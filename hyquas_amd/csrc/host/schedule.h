@@ -44,9 +44,19 @@ struct GateGroup {
     static qindex newRelated(qindex old, const Gate& g, qindex localQubits, bool enableGlobal);
     void addGate(const Gate& g, int index, qindex localQubits, bool enableGlobal);
 
-    State initState(const State& oldState, int numLocalQubits, int minMatSize);
+    // inPlacePrecision >= 0: a BLAS group the in-place dense kernel can run
+    // (matrices of 4..7 qubits, a state of at least one tile in that
+    // precision) keeps the layout (identity cuttPerm); otherwise, and with
+    // -1, the reference's transpose brings its qubits to the lowest positions.
+    State initState(const State& oldState, int numLocalQubits, int minMatSize, int inPlacePrecision = -1);
     State initPerGateState(const State& oldState);
-    State initBlasState(const State& oldState, int numLocalQubits, int minMatSize);
+    State initBlasState(const State& oldState, int numLocalQubits, int minMatSize, int inPlacePrecision = -1);
+    // Logical qubit of each matrix index bit (BLAS): the group's qubits and,
+    // up to matQubit, the lowest-placed other qubits, in order of their
+    // position in `state`.  After the reference's transpose these are the
+    // qubits at positions 0..matQubit-1 (bit j = state.layout[j]); a group
+    // run in place finds them where they are.
+    std::vector<int> matLogical() const;
     // FP64 fusion of the group into `matrix` (src/schedule.cpp:578-702)
     void initCPUMatrix();
 };
@@ -59,8 +69,11 @@ struct LocalGroup {
 
 struct Schedule {
     std::vector<LocalGroup> localGroups;   // one on a single GPU
-    State finalState;
+    State finalState;                      // the partition's final layout (serialized)
+    State runState;                        // the layout a run leaves (set when the executor lowers the groups)
     bool initFused = false;                // the first group's program builds |0...0> itself
+    // the layout of the state after a run: runState once lowered, else the partition's
+    const State& stateLayout() const { return runState.pos.empty() ? finalState : runState; }
     void dump(int numQubits) const;        // SHOW_SCHEDULE print (src/schedule.cpp:66-154)
     void clear();
     // Byte form (src/schedule.cpp:156-325); deserialize throws on malformed input

@@ -162,9 +162,16 @@ def _logical_topk(idx, prob, pos):
     return out, list(prob)
 
 
+# From this many qubits every TransMM group runs in place (matrices of 4..7
+# qubits, either precision), so resident states need no work buffer.
+INPLACE_MIN_QUBITS = 12
+
+
 def resident_states(hq, requests):
-    """One complex128 state with a work buffer per qubit count of `requests`."""
-    return {n: hq.State(n, hq.HQ_C128, with_buffer=True) for n in sorted({r["n"] for r in requests})}
+    """One complex128 state per qubit count of `requests` (with a work buffer
+    below INPLACE_MIN_QUBITS, where TransMM groups may run out of place)."""
+    return {n: hq.State(n, hq.HQ_C128, with_buffer=n < INPLACE_MIN_QUBITS)
+            for n in sorted({r["n"] for r in requests})}
 
 
 def serve_hybrid(hq, requests, circuits=None, states=None):

@@ -156,6 +156,17 @@ hq_status hq_program_create_sharemem(hq_state s, const hq_gate *gates, int num_g
 hq_status hq_program_create_blas(hq_state s, const int *perm, const double *mat_colmajor,
                                  int mat_qubits, hq_program *out);
 
+/* A TransMM group's matrix applied where its qubits are, in one pass and in
+ * place: no transpose, no work buffer, the layout unchanged.  Replaces the
+ * cuttPlan + cublasZgemm pair above (src/executor.cpp:533-551) when the
+ * caller does not need the reference's permuted layout.  Matrix index bit j
+ * (row and column, same format as hq_program_create_blas) is the state's
+ * qubit positions[j]; mat_qubits in 4..7 and a state of at least one tile:
+ * mat_qubits + 5 qubits in complex128 (+ 4 at 7), + 6 in complex64 (+ 5 at
+ * 7); 12 qubits always suffice. */
+hq_status hq_program_create_dense(hq_state s, const int *positions, const double *mat_colmajor,
+                                  int mat_qubits, hq_program *out);
+
 /* Launch a program on a state (launchExecutor, src/kernelOpt.cu:503-506, or
  * applyBlasGroup, src/executor.cpp:533-551).  Asynchronous. */
 hq_status hq_program_launch(hq_state s, hq_program p);
@@ -271,6 +282,14 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *                 one: 0 off, 1 on, 2 the whole-line kernel everywhere
  *                 (untimed), -1 auto (default: on with specialised kernels
  *                 from 26 qubits); timing overwrites the state
+ *   "transmm_inplace" TransMM groups: 1 (default) planned and run as one
+ *                 in-place dense pass over the group's qubits where they are
+ *                 (no transpose, no work buffer, the layout unchanged, so later
+ *                 OShareMem tiles coalesce on the qubits actually lowest); 0
+ *                 the reference's transpose + GEMM and permuted layouts
+ *                 (src/schedule.cpp:339-382, src/executor.cpp:533-551;
+ *                 "profile" = 1 sets 0); groups with matrices beyond 7 qubits
+ *                 or states below one dense tile use the transpose form
  *   "profile_events" 1 = time every launch with HIP events
  * Must be set before compile. */
 hq_status hq_circuit_set_option(hq_circuit c, const char *key, int64_t value);
@@ -321,13 +340,17 @@ hq_status hq_circuit_state(hq_circuit c, hq_state *out);
 /* Run c on a state the caller owns (a resident state reused by circuit after
  * circuit) instead of one that run() allocates: s must have c's qubit count
  * and, at compile/run time, c's precision and a work buffer if the schedule
- * has TransMM groups.  c borrows s: run(destroy=1) and hq_circuit_destroy
+ * has TransMM groups that run out of place (the transpose form).  c borrows s: run(destroy=1) and hq_circuit_destroy
  * leave it allocated, and s must outlive its use by c.  s = NULL returns c to
  * states of its own.  No counterpart in the reference, whose Circuit
  * allocates its state in run() (src/circuit.cpp:16-71). */
 hq_status hq_circuit_use_state(hq_circuit c, hq_state s);
 
-/* Final logical->physical map (schedule.finalState.pos), pos[n]. */
+/* Logical->physical map of the state a run leaves, pos[n]: after compile the
+ * layout the lowered groups produce (TransMM groups in place keep the layout;
+ * in the transpose form, option "transmm_inplace" = 0, they leave the
+ * partition's permuted layout, schedule.finalState.pos of the reference);
+ * after partition only, the partition's final layout. */
 hq_status hq_circuit_final_pos(hq_circuit c, int *pos);
 
 /* Schedule summary after compile: number of groups, and per group its

@@ -271,3 +271,17 @@ def flat_cost_tables(blas_ms, gate_cost=512.0, cutt=0.0):
     rows = ["1"] + [str(gate_cost)] * 14 + [""] + [str(gate_cost)] * 7 + [""]
     rows += [f"{1 << i} {blas_ms}" for i in range(10)] + ["", str(cutt)]
     return "\n".join(rows) + "\n"
+
+
+def dense_at_reference(psi, n, positions, U):
+    """U (K x K, matrix index bit j = qubit positions[j]) applied to psi,
+    composed from the oracle: transpose those qubits to the lowest bits in
+    that order, apply U to every run of K amplitudes, transpose back."""
+    m = len(positions)
+    perm = list(positions) + [q for q in range(n) if q not in positions]   # new bit i = old bit perm[i]
+    inv = [0] * n
+    for i, q in enumerate(perm):
+        inv[q] = i
+    x = oracle.transpose(psi, n, perm)
+    x = oracle.dense_runs(x, n, np.asarray(U).T.reshape(-1), m)
+    return oracle.transpose(x, n, inv)

@@ -41,13 +41,15 @@ def test_modes_match_oracle(gpu, n, mode):
     assert rel_err(got, want) <= 1e-10
 
 
-@pytest.mark.parametrize("n,jit,tune", [(20, 1, 0), (20, 1, 1), (20, 0, 0), (22, 1, 0)])
-def test_mixed_schedule_interleaves_transmm_and_sharemem(gpu, tmp_path, n, jit, tune):
+@pytest.mark.parametrize("n,jit,tune,inplace", [(20, 1, 0, 0), (20, 1, 1, 0), (20, 0, 0, 0), (22, 1, 0, 0),
+                                                (20, 1, 0, 1), (20, 0, 0, 1), (22, 1, 1, 1)])
+def test_mixed_schedule_interleaves_transmm_and_sharemem(gpu, tmp_path, n, jit, tune, inplace):
     """A schedule whose groups alternate between OShareMem and TransMM, chosen
     per group by cost per gate (src/compiler.cpp:248-279) under tables that
     make a dense group pay off where gates crowd onto six qubits: OShareMem
-    groups lowered through the layouts the TransMM permutations leave, zero-tile
-    tracking stopped at the first TransMM group, the logical readout after the
+    groups lowered through the layouts the TransMM permutations leave (or,
+    with TransMM in place, through the unchanged layout), zero-tile tracking
+    stopped at the first TransMM group, the logical readout after the
     sequence; complex128 against the per-gate oracle at 1e-10."""
     spec = blocks_and_layers(n)
     f = tmp_path / "cost.out"
@@ -56,14 +58,21 @@ def test_mixed_schedule_interleaves_transmm_and_sharemem(gpu, tmp_path, n, jit, 
     c.set_option("mode", gpu.HQ_MODE_MIX)
     c.set_option("jit", jit)
     c.set_option("tune", tune)
+    c.set_option("transmm_inplace", inplace)
     c.load_cost_model(str(f))
     c.compile()
     sched = c.schedule()
     kinds = "".join("B" if g["backend"] == gpu.HQ_BACKEND_BLAS else "P" for g in sched)
-    print(n, jit, tune, kinds, flush=True)
+    print(n, jit, tune, inplace, kinds, [round(g["live_fraction"], 4) for g in sched], flush=True)
     assert kinds.count("B") >= 2 and "BP" in kinds and "PB" in kinds
     first_b = kinds.index("B")
-    assert all(g["live_fraction"] == 1.0 for g in sched[first_b + 1:])
+    if inplace:
+        # no transpose moved a qubit; zero tiles are still tracked past the
+        # TransMM groups (their qubits count as touched)
+        assert c.final_pos() == list(range(n))
+        assert all(0 < g["live_fraction"] <= 1.0 for g in sched)
+    else:
+        assert all(g["live_fraction"] == 1.0 for g in sched[first_b + 1:])
     if first_b > 0:
         assert sched[0]["live_fraction"] == 0.5
     assert sum(g["num_gates"] for g in sched) == len(spec)
@@ -79,7 +88,8 @@ def test_mixed_schedule_interleaves_transmm_and_sharemem(gpu, tmp_path, n, jit, 
 @pytest.mark.parametrize("case", sorted(MIX_CASES))
 def test_hand_traced_mix_schedules_run(gpu, tmp_path, case):
     """The hand-traced MIX-mode partitions of the oracle's tests, run on the
-    GPU: a TransMM group, then OShareMem groups on the permuted layout."""
+    GPU ("profile" = 1: the reference's transposes): a TransMM group, then
+    OShareMem groups on the permuted layout."""
     n = 12
     c = _circuit(gpu, n, MIX_SPEC)
     c.set_option("profile", 1)

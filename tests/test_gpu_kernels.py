@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import H, group_for_mask as _group_for_mask, rand_state as _rand_state, rel_err, random_mixed, \
+from helpers import dense_at_reference, H, group_for_mask as _group_for_mask, rand_state as _rand_state, rel_err, random_mixed, \
     to_oracle
 
 pytestmark = pytest.mark.gpu
@@ -151,6 +151,58 @@ def test_dense_runs_many_tiles_per_block(gpu, prec, mq):
     s.upload(psi0.astype(np.complex64) if c64 else psi0)
     s.apply_dense(colmajor, mq)
     assert rel_err(s.download(), want) <= (2e-6 if c64 else 1e-12)
+    s.close()
+
+
+@pytest.mark.parametrize("prec,n,mq,seed", [
+    ("c128", 13, 4, 1), ("c128", 13, 5, 2), ("c128", 14, 6, 3), ("c128", 13, 7, 4),
+    ("c128", 24, 6, 5), ("c128", 24, 7, 6), ("c128", 22, 4, 7),
+    ("c64", 14, 4, 8), ("c64", 14, 6, 9), ("c64", 13, 7, 10), ("c64", 24, 6, 11), ("c64", 24, 7, 12)])
+def test_dense_in_place_matches_oracle(gpu, prec, n, mq, seed):
+    """hq_program_create_dense: a TransMM group's matrix on qubits at
+    arbitrary (unsorted) positions, in place and without a work buffer,
+    against the oracle's transpose + dense + inverse transpose; 24 qubits
+    give several tiles per block (the pipelined path), 13-14 the one-tile
+    minimum; positions include and exclude the lowest qubits."""
+    rs = np.random.default_rng(seed)
+    positions = [int(x) for x in rs.choice(n, mq, replace=False)]
+    if seed % 3 == 0 and 0 not in positions:
+        positions[0] = 0
+    K = 1 << mq
+    U = _rand_unitary(K, 400 + seed)
+    psi0 = _rand_state(n, 500 + seed)
+    want = dense_at_reference(psi0, n, positions, U)
+    c64 = prec == "c64"
+    s = gpu.State(n, gpu.HQ_C64 if c64 else gpu.HQ_C128)
+    s.upload(psi0.astype(np.complex64) if c64 else psi0)
+    p = s.program_dense(positions, U.T.reshape(-1))
+    s.launch(p)
+    s.sync()
+    got = s.download()
+    print(prec, n, mq, positions, rel_err(got, want), flush=True)
+    assert rel_err(got, want) <= (2e-6 if c64 else 1e-12)
+    # a second launch applies U again
+    s.launch(p)
+    s.sync()
+    want2 = dense_at_reference(want, n, positions, U)
+    assert rel_err(s.download(), want2) <= (4e-6 if c64 else 1e-12)
+    p.close()
+    s.close()
+
+
+def test_dense_in_place_rejects_bad_arguments(gpu):
+    s = gpu.State(13)
+    U = np.eye(16)
+    with pytest.raises(RuntimeError):
+        s.program_dense([0, 1, 2, 2], U)         # repeated qubit
+    with pytest.raises(RuntimeError):
+        s.program_dense([0, 1, 2, 13], U)        # outside the state
+    with pytest.raises(RuntimeError):
+        s.program_dense([0, 1, 2], np.eye(8))    # below 4 matrix qubits
+    small = gpu.State(8)
+    with pytest.raises(RuntimeError):
+        small.program_dense([0, 1, 2, 3], U)     # smaller than one tile
+    small.close()
     s.close()
 
 

@@ -388,3 +388,34 @@ def test_qasm_writer_round_trips_through_the_reader(hq):
         assert (a["type"], a["target"], a["control"], a["control2"]) == \
                (b["type"], b["target"], b["control"], b["control2"])
         assert np.array_equal(a["mat"], b["mat"])
+
+
+@pytest.mark.parametrize("n,prec", [(20, 0), (22, 1)])
+def test_transmm_in_place_planning(hq, tmp_path, n, prec):
+    """Option "transmm_inplace": with it (the default) the partitioner plans
+    TransMM groups that keep the layout (the final layout is the identity),
+    without it the reference's transposes move the group's qubits to the
+    lowest positions; both partitions hold every gate once and interleave
+    TransMM and OShareMem groups under tables that favour dense groups."""
+    from helpers import blocks_and_layers, flat_cost_tables
+    spec = blocks_and_layers(n)
+    f = tmp_path / "cost.out"
+    f.write_text(flat_cost_tables(0.03))
+    out = {}
+    for inplace in (1, 0):
+        c = hq.Circuit(n)
+        for s in spec:
+            c.addGate(hq.Gate.make(s[0], s[4] if len(s) > 4 else -1, s[2], s[1], s[3] if len(s) > 3 else ()))
+        c.set_option("mode", hq.HQ_MODE_MIX)
+        c.set_option("precision", prec)
+        c.set_option("transmm_inplace", inplace)
+        c.load_cost_model(str(f))
+        c.partition()
+        sched = c.schedule()
+        kinds = "".join("B" if g["backend"] == hq.HQ_BACKEND_BLAS else "P" for g in sched)
+        assert kinds.count("B") >= 2 and "BP" in kinds
+        assert sum(g["num_gates"] for g in sched) == len(spec)
+        assert sorted(c.schedule_order()) == list(range(len(spec)))
+        out[inplace] = c.final_pos()
+    assert out[1] == list(range(n))
+    assert out[0] != list(range(n)) and sorted(out[0]) == list(range(n))

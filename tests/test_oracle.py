@@ -459,3 +459,26 @@ def test_inverse_helper_is_the_adjoint():
     assert rel_err(out, psi0) <= 1e-13
     half = oracle.run_plain(to_oracle(spec), n, psi0)
     assert rel_err(half, psi0) > 0.1
+
+
+@pytest.mark.parametrize("n,positions", [(8, [5, 1, 6, 3]), (7, [0, 6, 2, 4, 1]), (9, [8, 7, 6, 5, 0, 1])])
+def test_dense_at_reference_matches_brute_force(n, positions):
+    """The reference for an in-place TransMM group (U on qubits where they
+    are: transpose, dense runs, transpose back) equals U scattered amplitude
+    by amplitude: out[x with matrix bits r] += U[r, k] psi[x with bits k]."""
+    from helpers import dense_at_reference, rand_state
+    m = len(positions)
+    K = 1 << m
+    rs = np.random.default_rng(n)
+    U = rs.normal(size=(K, K)) + 1j * rs.normal(size=(K, K))
+    psi = rand_state(n, n + 1)
+    got = dense_at_reference(psi, n, positions, U)
+    want = np.zeros_like(psi)
+    mask = sum(1 << p for p in positions)
+    for x in range(1 << n):
+        k = sum(((x >> p) & 1) << j for j, p in enumerate(positions))
+        base = x & ~mask
+        for r in range(K):
+            y = base | sum(((r >> j) & 1) << p for j, p in enumerate(positions))
+            want[y] += U[r, k] * psi[x]
+    assert np.max(np.abs(got - want)) <= 1e-12
