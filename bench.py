@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--batch-jit", type=int, choices=[0, 1], default=1,
                     help="batch: 1 specialised batched kernels shared by same-structure groups (default), "
                          "0 the interpreter kernel with autotuned tiles")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="batch/serve: streams a round's group launches spread over (0: engine default, 4)")
     ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
     ap.add_argument("--rates", default="20,100", help="serve: arrival rates (requests/s), comma separated")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain per-gate baseline")
@@ -405,6 +407,8 @@ def batch_form(a, hq, count, plain_sample, precisions=("c64", "c128")):
     for name in precisions:
         prec = hq.HQ_C64 if name == "c64" else hq.HQ_C128
         e = hq.Engine(prec, 280 * 10 ** 9)
+        if a.streams:
+            e.set_option("streams", a.streams)
         t0 = time.perf_counter()
         if a.batch_jit:
             # specialised batched kernels (one per group structure) on the
@@ -496,6 +500,8 @@ def serve_form(a, hq, rate, requests, seed, plain=True):
         serve_plain, summarize
     reqs = request_stream(requests, rate, seed)
     e = hq.Engine(hq.HQ_C64, 280 * 10 ** 9)
+    if a.streams:
+        e.set_option("streams", a.streams)
     circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in reqs]
     recs, wall = serve(hq, e, reqs, circuits)
     st = e.stats()

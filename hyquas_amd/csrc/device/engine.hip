@@ -350,7 +350,38 @@ struct EngineOptions {
     int compile_threads = 4;
     int max_inflight = 2;     // rounds queued on the device ahead of the host
     int jit = 0;              // 1: specialised batched kernels, shared by same-structure groups
+    int streams = 4;          // streams a round's group launches spread over (they touch different circuits)
     int tile_by_n[65] = {0};  // autotuned tile per qubit count (0: the rule above)
+};
+
+// Streams a round's group launches fork onto from the engine stream and join
+// back from (events), so launches of different group structures, each often
+// a few blocks over small states, can run side by side.
+struct SideStreams {
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hq_status create(int n)
+    {
+        if (!streams.empty() || n <= 1) return HQ_OK;
+        HQ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        for (int i = 0; i < n; i++) {
+            hipStream_t st = nullptr;
+            HQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));   // default priority
+            streams.push_back(st);
+            hipEvent_t ev = nullptr;
+            HQ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            join.push_back(ev);
+        }
+        return HQ_OK;
+    }
+    ~SideStreams()
+    {
+        for (auto st : streams) (void)hipStreamSynchronize(st);
+        for (auto st : streams) (void)hipStreamDestroy(st);
+        for (auto ev : join) (void)hipEventDestroy(ev);
+        if (fork) (void)hipEventDestroy(fork);
+    }
 };
 
 struct GroupPlan {
@@ -725,12 +756,41 @@ static void build_round(const std::vector<Job*>& jobs, uint64_t res_dev, RoundBu
     }
 }
 
-static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int precision, hipStream_t st)
+static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int precision, hipStream_t main_st,
+                             SideStreams* side = nullptr)
 {
+    // the group launches of a round touch different circuits: with side
+    // streams they fork from the engine stream, spread round-robin, and join
+    // back before the round's finish and top-k launches (which read them)
+    size_t ngroup = 0;
+    for (const LaunchDesc& d : rb.launches) ngroup += d.kind == 0 || d.kind == 3;
+    const bool fork = side && !side->streams.empty() && ngroup > 1;
+    const size_t nside = fork ? std::min(side->streams.size(), ngroup) : 0;
+    bool joined = !fork;
+    auto join = [&]() -> hq_status {
+        for (size_t k = 0; k < nside; k++) {
+            HQ_HIP(hipEventRecord(side->join[k], side->streams[k]));
+            HQ_HIP(hipStreamWaitEvent(main_st, side->join[k], 0));
+        }
+        joined = true;
+        return HQ_OK;
+    };
+    if (fork) {
+        HQ_HIP(hipEventRecord(side->fork, main_st));
+        for (size_t k = 0; k < nside; k++) HQ_HIP(hipStreamWaitEvent(side->streams[k], side->fork, 0));
+    }
+    size_t next_side = 0;
     for (const LaunchDesc& d : rb.launches) {
         const void* tab = (const void*)(uintptr_t)(table_dev + d.table_off);
         HQ_CHECK(d.blocks < (1ull << 31), HQ_ERR_UNSUPPORTED, "engine: launch too large");
         hq_status rc = HQ_OK;
+        const bool group = d.kind == 0 || d.kind == 3;
+        if (!group && !joined) {
+            rc = join();
+            if (rc != HQ_OK) return rc;
+        }
+        hipStream_t st = main_st;
+        if (group && fork) st = side->streams[next_side++ % nside];
         if (d.kind == 0) {
             rc = precision == HQ_C128
                      ? launch_batch_shape<double>(d.rb, d.tb, (const BatchItem*)tab, d.count, d.blocks, st)
@@ -754,6 +814,7 @@ static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int preci
         }
         if (rc != HQ_OK) return rc;
     }
+    if (!joined) return join();
     return HQ_OK;
 }
 
@@ -852,6 +913,7 @@ struct hq_engine_s {
     KernelCache kernels;               // specialised batched kernels (option "jit")
     Arena arena;
     hipStream_t stream = nullptr;
+    SideStreams side;                  // created at the first round (option "streams")
     double epoch = 0;
 
     std::mutex mu;                     // jobs map, compile queue
@@ -926,6 +988,7 @@ struct hq_engine_s {
         cv.notify_all();
         for (auto& t : workers) t.join();
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto st : side.streams) (void)hipStreamSynchronize(st);
         for (auto& s : slots) slot_free(s);
         arena.release_all();
         if (stream) (void)hipStreamDestroy(stream);
@@ -1054,7 +1117,9 @@ hq_status launch_round(hq_engine e, const std::vector<Job*>& admitted)
     }
     memcpy(s.host + table_off, rb.table.data(), rb.table.size());
     HQ_HIP(hipMemcpyAsync(s.dev + table_off, s.host + table_off, rb.table.size(), hipMemcpyHostToDevice, e->stream));
-    rc = issue_round(rb, (uint64_t)(uintptr_t)s.dev + table_off, e->precision, e->stream);
+    rc = e->side.create(e->opt.streams);
+    if (rc != HQ_OK) return rc;
+    rc = issue_round(rb, (uint64_t)(uintptr_t)s.dev + table_off, e->precision, e->stream, &e->side);
     if (rc != HQ_OK) return rc;
     if (!rb.finishing.empty())
         HQ_HIP(hipMemcpyAsync(s.host + res_off, s.dev + res_off, rb.finishing.size() * RES_BYTES,
@@ -1168,6 +1233,10 @@ hq_status hq_engine_set_option(hq_engine e, const char* key, int64_t value)
     } else if (k == "max_inflight") {
         HQ_CHECK(value >= 1 && value <= 8, HQ_ERR_ARG, "max_inflight must be in [1, 8]");
         e->opt.max_inflight = (int)value;
+    } else if (k == "streams") {
+        HQ_CHECK(value >= 1 && value <= 8, HQ_ERR_ARG, "streams must be in [1, 8]");
+        HQ_CHECK(e->side.streams.empty(), HQ_ERR_ARG, "streams must be set before the first round or batch");
+        e->opt.streams = (int)value;
     } else {
         set_error("hq_engine_set_option: unknown option " + k);
         return HQ_ERR_ARG;
@@ -1427,10 +1496,12 @@ hq_status hq_batch_create(hq_engine e, const hq_circuit* circuits, int count, in
     // the result of circuit i is at job_res[i]; keep the mapping in next_group
     for (int i = 0; i < count; i++) b->jobs[i]->next_group = (int)job_res[i];
     HQ_HIP(hipStreamSynchronize(e->stream));
+    hq_status rc = e->side.create(e->opt.streams);
+    if (rc != HQ_OK) return rc;
     HQ_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    hq_status rc = HQ_OK;
     for (size_t r = 0; r < builds.size() && rc == HQ_OK; r++) {
-        rc = issue_round(builds[r], (uint64_t)(uintptr_t)b->d_table + round_off[r], e->precision, e->stream);
+        rc = issue_round(builds[r], (uint64_t)(uintptr_t)b->d_table + round_off[r], e->precision, e->stream,
+                         &e->side);
         b->launches += (int)builds[r].launches.size();
     }
     hipError_t ce = hipMemcpyAsync(b->h_res, b->d_res, (size_t)count * RES_BYTES, hipMemcpyDeviceToHost, e->stream);

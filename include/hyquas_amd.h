@@ -417,7 +417,10 @@ enum { HQ_JOB_QUEUED = 0, HQ_JOB_COMPILED = 1, HQ_JOB_ADMITTED = 2, HQ_JOB_DONE 
 hq_status hq_engine_create(int precision, uint64_t hbm_budget, hq_engine *out);
 hq_status hq_engine_destroy(hq_engine e);
 /* "whole_max" (13), "share_qubits" (12), "coalesce" (4), "compile_threads" (4,
- * before the first submit), "max_inflight" (2 rounds queued ahead of the host). */
+ * before the first submit), "max_inflight" (2 rounds queued ahead of the host),
+ * "jit" (0: interpreter; 1: specialised batched kernels), "streams" (4: the
+ * streams a round's group launches spread over, before the first round or
+ * batch; 1: one stream). */
 hq_status hq_engine_set_option(hq_engine e, const char *key, int64_t value);
 /* Specialised batched kernels the engine has compiled so far (option "jit";
  * one per distinct group structure).  Safe to call while another thread

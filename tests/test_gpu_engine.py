@@ -161,6 +161,35 @@ def test_batch_graph_c64_large_circuits(gpu):
     e.close()
 
 
+@pytest.mark.parametrize("jit", [0, 1])
+def test_side_streams_give_identical_results(gpu, jit):
+    """Option "streams": a round's group launches spread over side streams
+    (fork and join by events, captured into the batch graph) give the same
+    results bit for bit as one stream, in a batch and in continuous batching."""
+    hq = gpu
+    specs = [(16, 6, 1), (17, 5, 2), (19, 4, 3), (14, 8, 4), (16, 6, 5), (20, 3, 6), (12, 5, 7), (18, 4, 8)]
+    out = {}
+    for streams in (1, 4):
+        e = hq.Engine(hq.HQ_C64, 1 << 31)
+        e.set_option("streams", streams)
+        e.set_option("jit", jit)
+        b = e.batch([_hea(hq, n, d, s) for n, d, s in specs], hq.HQ_OUT_EXPECT | hq.HQ_OUT_TOPK)
+        b.run()
+        r, t = b.results()
+        b.run()
+        r2, t2 = b.results()
+        assert np.array_equal(r, r2) and np.array_equal(t[1], t2[1])
+        b.close()
+        jobs = [e.submit(_rl(hq, n, d, s), hq.HQ_OUT_EXPECT) for n, d, s in specs]
+        e.drain()
+        cont = [e.result(j)["z"] for j in jobs]
+        e.close()
+        out[streams] = (r, t, cont)
+    assert np.array_equal(out[1][0], out[4][0])
+    assert np.array_equal(out[1][1][0], out[4][1][0]) and np.array_equal(out[1][1][1], out[4][1][1])
+    assert out[1][2] == out[4][2]
+
+
 def test_batch_pass_bytes(gpu):
     """hq_batch_pass_bytes: a whole-state circuit is one write-only pass; a
     partitioned one writes once, then reads and writes at most the whole
