@@ -552,8 +552,8 @@ def main_serve(a, hq):
 
 
 SERVE_WORKLOAD = ("Poisson stream, {requests} requests per rate, random-layer or ansatz (even odds), qubits "
-                  "round(exp(U[ln12, ln28])), depth U[8, 60], output sum <Z_i> or top-16 (even odds), seed {seed}, "
-                  "HBM cap 280 GB")
+                  "round(exp(U[ln12, ln28])), depth U[8, 60], output sum <Z_i> or top-16 (even odds), seed {seed} + k "
+                  "for the k-th rate, HBM cap 280 GB")
 
 
 def _timed_run(hq, c, steps, warmup):
