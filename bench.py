@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--batch-jit", type=int, choices=[0, 1], default=1,
                     help="batch: 1 specialised batched kernels shared by same-structure groups (default), "
                          "0 the interpreter kernel with autotuned tiles")
+    ap.add_argument("--batch-tile", type=int, default=0,
+                    help="batch: tile qubits of partitioned circuits (0: engine default, 12)")
     ap.add_argument("--streams", type=int, default=0,
                     help="batch/serve: streams a round's group launches spread over (0: engine default, 4)")
     ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
@@ -409,10 +411,12 @@ def batch_form(a, hq, count, plain_sample, precisions=("c64", "c128")):
         e = hq.Engine(prec, 280 * 10 ** 9)
         if a.streams:
             e.set_option("streams", a.streams)
+        if a.batch_tile:
+            e.set_option("share_qubits", a.batch_tile)
         t0 = time.perf_counter()
         if a.batch_jit:
             # specialised batched kernels (one per group structure) on the
-            # default tiles: the whole state up to 13 qubits, else 12
+            # default tiles: the whole state up to 13 qubits, else 12 (13 in complex128)
             e.set_option("compile_threads", 16)
             e.set_option("jit", 1)
             tiles = {}

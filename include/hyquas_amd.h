@@ -416,7 +416,7 @@ enum { HQ_JOB_QUEUED = 0, HQ_JOB_COMPILED = 1, HQ_JOB_ADMITTED = 2, HQ_JOB_DONE 
 
 hq_status hq_engine_create(int precision, uint64_t hbm_budget, hq_engine *out);
 hq_status hq_engine_destroy(hq_engine e);
-/* "whole_max" (13), "share_qubits" (12), "coalesce" (4), "compile_threads" (4,
+/* "whole_max" (13), "share_qubits" (12; 13 with "jit" in complex128), "coalesce" (4), "compile_threads" (4,
  * before the first submit), "max_inflight" (2 rounds queued ahead of the host),
  * "jit" (0: interpreter; 1: specialised batched kernels), "streams" (4: the
  * streams a round's group launches spread over, before the first round or
