@@ -126,7 +126,7 @@ def main():
                     help="batch: 1 specialised batched kernels shared by same-structure groups (default), "
                          "0 the interpreter kernel with autotuned tiles")
     ap.add_argument("--batch-tile", type=int, default=0,
-                    help="batch: tile qubits of partitioned circuits (0: engine default, 12)")
+                    help="batch/serve: engine tile qubits of partitioned circuits (0: engine default)")
     ap.add_argument("--streams", type=int, default=0,
                     help="batch/serve: streams a round's group launches spread over (0: engine default, 4)")
     ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
@@ -506,6 +506,8 @@ def serve_form(a, hq, rate, requests, seed, plain=True):
     e = hq.Engine(hq.HQ_C64, 280 * 10 ** 9)
     if a.streams:
         e.set_option("streams", a.streams)
+    if a.batch_tile:
+        e.set_option("share_qubits", a.batch_tile)
     circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in reqs]
     recs, wall = serve(hq, e, reqs, circuits)
     st = e.stats()
