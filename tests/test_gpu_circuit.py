@@ -384,3 +384,33 @@ def test_circuits_share_a_resident_state(gpu):
     c.close()
     small.close()
     st.close()
+
+
+def test_in_place_transmm_needs_no_work_buffer(gpu, tmp_path):
+    """A mixed schedule whose TransMM groups run in place runs on a borrowed
+    state without a work buffer (against the oracle); the same circuit in the
+    reference's transpose form asks for the buffer and refuses that state."""
+    n = 20
+    spec = blocks_and_layers(n)
+    f = tmp_path / "cost.out"
+    f.write_text(flat_cost_tables(0.03))
+    want = oracle.run_plain(to_oracle(spec), n)
+    st = gpu.State(n, gpu.HQ_C128)            # no work buffer
+    c = _circuit(gpu, n, spec)
+    c.set_option("mode", gpu.HQ_MODE_MIX)
+    c.load_cost_model(str(f))
+    c.use_state(st)
+    c.compile()
+    assert any(g["backend"] == gpu.HQ_BACKEND_BLAS for g in c.schedule())
+    c.run(copy_back=False, destroy=False)
+    assert rel_err(c.logical_state(), want) <= 1e-10
+    c.close()
+    c = _circuit(gpu, n, spec)
+    c.set_option("mode", gpu.HQ_MODE_MIX)
+    c.set_option("transmm_inplace", 0)
+    c.load_cost_model(str(f))
+    c.use_state(st)
+    with pytest.raises(gpu.HQError, match="work buffer"):
+        c.compile()
+    c.close()
+    st.close()
