@@ -173,7 +173,7 @@ hq_status hq_circuit_set_option(hq_circuit c, const char* key, int64_t v)
         C.params.starts = (int)v;
         C.beamAuto = false;
     } else if (k == "beam") {
-        HQ_CARG(v >= 1 && v <= 64, "beam must be in [1, 64]");
+        HQ_CARG(v >= 1 && v <= 1024, "beam must be in [1, 1024]");
         C.params.beam = (int)v;
         C.beamAuto = false;
     } else if (k == "jit") {
