@@ -414,3 +414,46 @@ def test_in_place_transmm_needs_no_work_buffer(gpu, tmp_path):
         c.compile()
     c.close()
     st.close()
+
+
+_JIT_CHILD = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import hyquas_amd as hq
+c = hq.Circuit(22).add_random_layers(4, 7)
+c.set_option("jit", 1)
+c.set_option("tune", 0)
+c.compile()
+c.run(copy_back=False, destroy=False)
+idx, prob = c.state_view().topk(4)
+print("OK", c.state_view().norm2(), " ".join(str(int(i)) for i in idx), " ".join(repr(float(p)) for p in prob))
+"""
+
+
+@pytest.mark.parametrize("helper", ["none", "/nonexistent/hq_rtc"])
+def test_specialised_kernels_without_compile_helper(gpu, tmp_path, helper):
+    """Specialised kernels still build when the compile helper is switched off
+    (HQ_RTC_HELPER=none: hiprtc in the process) or cannot start (a missing
+    path: one warning, then in-process compiles), with a cold kernel cache;
+    the results equal the helper-built kernels' bit for bit."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = {}
+    for name, env_helper in (("helper", None), ("alt", helper)):
+        # cold caches: ours and the compiler library's own
+        env = dict(os.environ, HQ_JIT_CACHE=str(tmp_path / name), AMD_COMGR_CACHE="0")
+        env.pop("HQ_RTC_HELPER", None)
+        if env_helper is not None:
+            env["HQ_RTC_HELPER"] = env_helper
+        r = subprocess.run([sys.executable, "-c", _JIT_CHILD, root], env=env, capture_output=True, text=True,
+                           timeout=240)
+        print(name, r.returncode, r.stdout[-300:], r.stderr[-300:], flush=True)
+        assert r.returncode == 0
+        line = [l for l in r.stdout.splitlines() if l.startswith("OK")][-1]
+        outs[name] = line
+        if name == "alt" and helper != "none":
+            assert "compile helper" in r.stderr and "in-process" in r.stderr
+    assert outs["helper"] == outs["alt"]
+    assert abs(float(outs["helper"].split()[1]) - 1) < 1e-12
