@@ -101,7 +101,8 @@ struct Emitter {
         split = ((size_t)1 << K) * (f32 ? 8 : 16) > 64 * 1024;
         // non-temporal HBM accesses: chosen per group (they pay only where a
         // wave instruction covers whole 128-byte lines)
-        nt_ld = nt_st = p.nt;
+        nt_ld = p.nt;
+        nt_st = p.nt || p.nt_store;
         const char* bm = getenv("HQ_JIT_BMAP");
         bmap = bm ? atoi(bm) : 0;
         if ((bmap == 2 && P.hdr.num_block_bits < 3) || (bmap == 1 && P.hdr.num_block_bits == 0) || batch) bmap = 0;
@@ -174,7 +175,7 @@ struct Emitter {
     // neighbours in HBM and move as one 16-byte access
     int pair_pos(const ShmStage& st) const
     {
-        if (!f32 || !P.nt) return -1;
+        if (!f32 || !P.whole_line) return -1;
         for (int i = 0; i < P.rb; i++)
             if (P.hdr.tile_phys[st.reg_bits[i]] == 1) return i;
         return -1;
@@ -907,15 +908,20 @@ extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gat
 {
     HQ_CHECK(n >= 1 && n <= 36, HQ_ERR_ARG, "hq_sharemem_source: num_qubits must be in [1, 36]");
     HQ_CHECK(precision == HQ_C128 || precision == HQ_C64, HQ_ERR_ARG, "hq_sharemem_source: bad precision");
-    HQ_CHECK(variant >= 0 && variant <= 31, HQ_ERR_ARG, "hq_sharemem_source: variant must be in [0, 31]");
+    HQ_CHECK(variant >= 0 && variant <= 63, HQ_ERR_ARG, "hq_sharemem_source: variant must be in [0, 63]");
     const bool whole_line = variant & 1, init = (variant & 2) != 0, batch = (variant & 4) != 0,
-               readout = (variant & 8) != 0, shared_code = (variant & 16) != 0;
+               readout = (variant & 8) != 0, shared_code = (variant & 16) != 0, nt_store = (variant & 32) != 0;
     HQ_CHECK(batch || !readout, HQ_ERR_ARG, "hq_sharemem_source: the readout fold needs the batched form");
     hq::ShmPlan plan;
     hq_status st = hq::plan_sharemem(n, precision, gates, num_gates, shared_mask, reg_qubits, &plan, init,
                                      whole_line ? 3 : 0, shared_code);
     if (st != HQ_OK) return st;
-    plan.nt = whole_line;
+    // bit 0 alone: whole-line stages with non-temporal loads and stores; with
+    // bit 5: whole-line stages, non-temporal stores only; bit 5 alone: the
+    // plain stages with non-temporal stores
+    plan.whole_line = whole_line;
+    plan.nt = whole_line && !nt_store;
+    plan.nt_store = nt_store;
     plan.init = init;
     std::vector<double> cf;
     std::string src = hq::jit_source(plan, &cf, batch, readout);

@@ -270,13 +270,12 @@ hq_status hq_apply_dense(hq_state s, const double* mat, int mat_qubits)
 namespace hq {
 
 // each variant of every group is launched `rounds` times, interleaved with
-// the other variant; its fastest launch counts
-static hq_status time_variants(hq_state s, const std::vector<hq_program>& a, const std::vector<hq_program>& b,
-                               int rounds, std::vector<float>* ta, std::vector<float>* tb)
+// the other variants; its fastest launch counts (t[v][i])
+static hq_status time_variants(hq_state s, const std::vector<std::vector<hq_program>>& built, int rounds,
+                               std::vector<std::vector<float>>* t)
 {
-    const size_t n = a.size();
-    ta->assign(n, 1e30f);
-    tb->assign(n, 1e30f);
+    const size_t nv = built.size(), n = nv ? built[0].size() : 0;
+    t->assign(nv, std::vector<float>(n, 1e30f));
     hipEvent_t e0, e1;
     HQ_HIP(hipEventCreate(&e0));
     hipError_t he = hipEventCreate(&e1);
@@ -290,17 +289,15 @@ static hq_status time_variants(hq_state s, const std::vector<hq_program>& a, con
     hq_status st = HQ_OK;
     for (int r = 0; r < rounds && st == HQ_OK; r++) {
         for (size_t i = 0; i < n && st == HQ_OK; i++) {
-            for (int v = 0; v < 2 && st == HQ_OK; v++) {
-                hq_program p = v == 0 ? a[i] : b[i];
+            for (size_t v = 0; v < nv && st == HQ_OK; v++) {
                 if ((he = hipEventRecord(e0, s->stream)) != hipSuccess) break;
-                st = hq_program_launch(s, p);
+                st = hq_program_launch(s, built[v][i]);
                 if (st != HQ_OK) break;
                 if ((he = hipEventRecord(e1, s->stream)) != hipSuccess) break;
                 if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
                 float ms = 0;
                 if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-                float& t = v == 0 ? (*ta)[i] : (*tb)[i];
-                t = std::min(t, ms);
+                (*t)[v][i] = std::min((*t)[v][i], ms);
             }
         }
         if (he != hipSuccess) {
@@ -317,10 +314,12 @@ static hq_status time_variants(hq_state s, const std::vector<hq_program>& a, con
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
                                   std::vector<hq_program>* out, int tune_mode)
 {
-    // tune_mode 1: both variants, timed; 2: the whole-line variant only
+    // tune_mode 1: every variant (build_sharemem_program's 0..3), timed; 2:
+    // the whole-line variant only
     bool tune = tune_mode == 1 && jit && !reqs.empty();
     const int only = tune_mode == 2 && jit ? 1 : 0;
-    const int variants = tune ? 2 : 1;
+    constexpr int kVariants = 4;
+    const int variants = tune ? kVariants : 1;
     std::vector<std::vector<hq_program>> built(variants, std::vector<hq_program>(reqs.size(), nullptr));
     auto destroy_all = [&]() {
         for (auto& v : built)
@@ -345,8 +344,9 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
         int threads = e && atoi(e) > 0 ? atoi(e) : (int)(hw ? hw : 4);
         hq_status st = jit_sharemem_many(plans, progs, std::min(threads, 16));
         if (st != HQ_OK && tune) {
-            // the plain kernels alone (the whole-line variant is an option, not a requirement)
-            for (auto p : built[1]) hq_program_destroy(p);
+            // the plain kernels alone (the other variants are options, not requirements)
+            for (size_t v = 1; v < built.size(); v++)
+                for (auto p : built[v]) hq_program_destroy(p);
             built.resize(1);
             tune = false;
             plans.resize(reqs.size());
@@ -356,13 +356,16 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
         if (st != HQ_OK) { destroy_all(); return st; }
     }
     if (tune) {
-        std::vector<float> t0, t1;
-        hq_status st = time_variants(s, built[0], built[1], 2, &t0, &t1);
+        std::vector<std::vector<float>> t;
+        hq_status st = time_variants(s, built, 2, &t);
         if (st != HQ_OK) { destroy_all(); return st; }
         for (size_t i = 0; i < reqs.size(); i++) {
-            const int keep = t1[i] < t0[i] ? 1 : 0;
+            size_t keep = 0;
+            for (size_t v = 1; v < built.size(); v++)
+                if (t[v][i] < t[keep][i]) keep = v;
             (*out)[i] = built[keep][i];
-            hq_program_destroy(built[1 - keep][i]);
+            for (size_t v = 0; v < built.size(); v++)
+                if (v != keep) hq_program_destroy(built[v][i]);
         }
         return HQ_OK;
     }

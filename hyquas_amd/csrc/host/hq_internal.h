@@ -11,18 +11,19 @@ namespace hq {
 void set_error(const std::string& msg);
 // hq_program_create_sharemem with an explicit register/thread split
 // (reg_qubits = 0: default for the tile size); init: the group builds
-// |0...0> in registers instead of loading the state; variant 1: whole-line
-// HBM stages (added stages without gates where needed) with non-temporal
-// loads and stores in the specialised kernel.
+// |0...0> in registers instead of loading the state; variant (specialised
+// kernel): 0 plain stages; 1 whole-line HBM stages (added stages without gates
+// where needed) with non-temporal loads and stores; 2 plain stages with
+// non-temporal stores; 3 whole-line stages with non-temporal stores.
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask,
                                  int reg_qubits, hq_program* out, bool init = false, int variant = 0,
                                  uint64_t zero_mask = 0);
 
 // Many OShareMem programs at once; with jit, every group also gets a
 // structure-specialised kernel, compiled in parallel.  tune_mode (jit only):
-// 1 compiles both program variants of every group, times them on the state
-// (its contents are overwritten) and keeps the faster; 2 uses the whole-line
-// variant for every group; 0 the plain one.
+// 1 compiles the four program variants of every group, times them on the
+// state (its contents are overwritten) and keeps the fastest; 2 uses the
+// whole-line variant for every group; 0 the plain one.
 struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;

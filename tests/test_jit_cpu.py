@@ -25,7 +25,9 @@ CLANG = "/opt/rocm/llvm/bin/clang++"
     (11, 10, 32, 0, 3, 1), (14, 13, 35, 0, 4, 1), (15, 13, 36, 0, 5, 1), (14, 13, 37, 1, 5, 1),
     (15, 14, 38, 1, 0, 0), (16, 14, 39, 1, 6, 1), (15, 14, 40, 1, 6, 1), (13, 12, 41, 1, 4, 1),
     # first group of a run (bit 1): builds |0...0>, other tiles only store zeros
-    (15, 13, 42, 0, 5, 2), (15, 13, 43, 0, 5, 3), (16, 14, 44, 1, 6, 3)])
+    (15, 13, 42, 0, 5, 2), (15, 13, 43, 0, 5, 3), (16, 14, 44, 1, 6, 3),
+    # non-temporal stores only (bit 5), on the plain and the whole-line stages
+    (14, 13, 45, 0, 5, 32), (15, 13, 46, 0, 5, 33), (16, 14, 47, 1, 6, 33), (15, 14, 48, 1, 6, 32)])
 def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, variant):
     rs = np.random.default_rng(seed)
     rest = rs.choice(np.arange(3, n), k - 3, replace=False)
@@ -35,7 +37,9 @@ def test_generated_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, var
     g = _group_for_mask(n, mask, 90, seed)
     src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, g, mask, prec, rb, variant)
     assert tile == k and (rb == 0 or rbits == rb)
-    if variant & 1:
+    if variant & 32:
+        assert "__builtin_nontemporal_store" in src and "__builtin_nontemporal_load" not in src
+    elif variant & 1:
         assert "__builtin_nontemporal_store" in src
         assert "__builtin_nontemporal_load" in src or variant & 2
     # non-temporal builtins and vector types (16-byte complex64 pairs) are
