@@ -134,8 +134,8 @@ struct ShmHeader {
 struct ShmPlan {
     int n = 0, precision = 0, k = 0, rb = 0, tb = 0;
     bool init = false;        // start from |0...0> instead of loading (first group of a run)
-    bool nt = false;          // specialised kernel: non-temporal HBM loads and stores
-    bool nt_store = false;    // specialised kernel: non-temporal HBM stores (loads as usual)
+    bool nt_load = false;     // specialised kernel: non-temporal HBM loads
+    bool nt_store = false;    // specialised kernel: non-temporal HBM stores
     bool whole_line = false;  // planned with the whole-line HBM stages (io = 3)
     uint64_t shared_mask = 0;
     ShmHeader hdr;

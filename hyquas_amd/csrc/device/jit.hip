@@ -101,8 +101,8 @@ struct Emitter {
         split = ((size_t)1 << K) * (f32 ? 8 : 16) > 64 * 1024;
         // non-temporal HBM accesses: chosen per group (they pay only where a
         // wave instruction covers whole 128-byte lines)
-        nt_ld = p.nt;
-        nt_st = p.nt || p.nt_store;
+        nt_ld = p.nt_load;
+        nt_st = p.nt_store;
         const char* bm = getenv("HQ_JIT_BMAP");
         bmap = bm ? atoi(bm) : 0;
         if ((bmap == 2 && P.hdr.num_block_bits < 3) || (bmap == 1 && P.hdr.num_block_bits == 0) || batch) bmap = 0;
@@ -920,8 +920,8 @@ extern "C" hq_status hq_sharemem_source(int n, int precision, const hq_gate* gat
     // bit 5: whole-line stages, non-temporal stores only; bit 5 alone: the
     // plain stages with non-temporal stores
     plan.whole_line = whole_line;
-    plan.nt = whole_line && !nt_store;
-    plan.nt_store = nt_store;
+    plan.nt_load = whole_line && !nt_store;
+    plan.nt_store = whole_line || nt_store;
     plan.init = init;
     std::vector<double> cf;
     std::string src = hq::jit_source(plan, &cf, batch, readout);

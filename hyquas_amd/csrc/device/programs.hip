@@ -314,12 +314,16 @@ static hq_status time_variants(hq_state s, const std::vector<std::vector<hq_prog
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
                                   std::vector<hq_program>* out, int tune_mode)
 {
-    // tune_mode 1: every variant (build_sharemem_program's 0..3), timed; 2:
-    // the whole-line variant only
+    // tune_mode 1: four variants of build_sharemem_program, timed: plain
+    // stages; whole-line stages with non-temporal loads and stores; each with
+    // non-temporal stores only.  (All eight combinations, non-temporal loads
+    // alone included, tuned worse: 0.1574-0.1584 s against 0.1558 s at 30
+    // qubits; two launches per candidate cannot tell eight apart.)  2: the
+    // whole-line variant with non-temporal loads and stores only
     bool tune = tune_mode == 1 && jit && !reqs.empty();
-    const int only = tune_mode == 2 && jit ? 1 : 0;
-    constexpr int kVariants = 4;
-    const int variants = tune ? kVariants : 1;
+    const int only = tune_mode == 2 && jit ? 7 : 0;
+    static constexpr int kTuned[] = {0, 7, 2, 3};
+    const int variants = tune ? 4 : 1;
     std::vector<std::vector<hq_program>> built(variants, std::vector<hq_program>(reqs.size(), nullptr));
     auto destroy_all = [&]() {
         for (auto& v : built)
@@ -329,7 +333,7 @@ hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>
     for (int v = 0; v < variants; v++) {
         for (size_t i = 0; i < reqs.size(); i++) {
             hq_status st = build_sharemem_program(s, reqs[i].gates.data(), (int)reqs[i].gates.size(), reqs[i].mask,
-                                                  reg_qubits, &built[v][i], reqs[i].init, tune ? v : only,
+                                                  reg_qubits, &built[v][i], reqs[i].init, tune ? kTuned[v] : only,
                                                   reqs[i].zero_mask);
             if (st != HQ_OK) { destroy_all(); return st; }
         }

@@ -414,17 +414,16 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
     HQ_CHECK(s && out, HQ_ERR_ARG, "sharemem: null argument");
     *out = nullptr;
     ShmPlan plan;
-    // variant 0: plain stages; 1: whole-line stages, non-temporal loads and
-    // stores; 2: plain stages, non-temporal stores; 3: whole-line stages,
-    // non-temporal stores
+    // variant bit 0: whole-line stages; bit 1: non-temporal stores; bit 2:
+    // non-temporal loads
     const bool whole_line = variant & 1;
     hq_status st = plan_sharemem(s->n, s->precision, gates, num_gates, shared_mask, rb_req, &plan, init,
                                  whole_line ? 3 : 0);
     if (st != HQ_OK) return st;
     plan.init = init;
     plan.whole_line = whole_line;
-    plan.nt = variant == 1;
-    plan.nt_store = variant >= 2;
+    plan.nt_store = (variant & 2) != 0;
+    plan.nt_load = (variant & 4) != 0;
     plan.hdr.zero_mask = init ? 0 : zero_mask;
     const auto& dst = plan.stages;
     const auto& ops = plan.ops;

@@ -12,18 +12,19 @@ void set_error(const std::string& msg);
 // hq_program_create_sharemem with an explicit register/thread split
 // (reg_qubits = 0: default for the tile size); init: the group builds
 // |0...0> in registers instead of loading the state; variant (specialised
-// kernel): 0 plain stages; 1 whole-line HBM stages (added stages without gates
-// where needed) with non-temporal loads and stores; 2 plain stages with
-// non-temporal stores; 3 whole-line stages with non-temporal stores.
+// kernel) bit 0: whole-line HBM stages (added stages without gates where
+// needed); bit 1: non-temporal stores; bit 2: non-temporal loads.
 hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates, uint64_t shared_mask,
                                  int reg_qubits, hq_program* out, bool init = false, int variant = 0,
                                  uint64_t zero_mask = 0);
 
 // Many OShareMem programs at once; with jit, every group also gets a
 // structure-specialised kernel, compiled in parallel.  tune_mode (jit only):
-// 1 compiles the four program variants of every group, times them on the
+// 1 compiles four program variants of every group (plain or whole-line
+// stages, non-temporal loads and stores or stores only), times them on the
 // state (its contents are overwritten) and keeps the fastest; 2 uses the
-// whole-line variant for every group; 0 the plain one.
+// whole-line variant with non-temporal loads and stores for every group; 0
+// the plain one.
 struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;
