@@ -27,7 +27,8 @@ time on resident states and
 the plain per-gate path (both on 128 evenly spaced circuits of the batch);
 "serve", a Poisson stream of 500 requests at each of --form-rates served by
 the complex64 engine against the FP64 hybrid path one circuit at a time
-(circuits/s, p50, p99 latency).  --form batch|serve|precision run one form
+(circuits/s, p50, p99 latency), and the plain per-gate path on the first
+100 requests of each stream.  --form batch|serve|precision run one form
 alone with more detail.
 """
 from __future__ import annotations
@@ -280,7 +281,7 @@ def main():
         rates = [float(x) for x in a.form_rates.split(",") if x]
         forms = {"batch": batch_form(a, hq, a.form_count, 128, precisions=("c64",))[0],
                  "serve": {"workload": SERVE_WORKLOAD.format(requests=a.requests, seed=a.seed),
-                           "sweep": [serve_form(a, hq, r, a.requests, a.seed + k, plain=False)
+                           "sweep": [serve_form(a, hq, r, a.requests, a.seed + k, plain=True, plain_sample=100)
                                      for k, r in enumerate(rates)]}}
 
     out = {
@@ -496,10 +497,12 @@ def main_batch(a, hq):
     print(json.dumps(out), flush=True)
 
 
-def serve_form(a, hq, rate, requests, seed, plain=True):
+def serve_form(a, hq, rate, requests, seed, plain=True, plain_sample=0):
     """Serving form at one arrival rate: the complex64 engine (continuous
     batching, HBM admission under 280 GB) against the FP64 hybrid path one
-    circuit at a time on resident states (and the plain per-gate path)."""
+    circuit at a time on resident states (and the plain per-gate path, on the
+    first `plain_sample` requests of the same stream when that is > 0: the
+    plain path runs a few circuits per second)."""
     from hyquas_amd.serving import build_circuit, request_stream, resident_states, serve, serve_hybrid, \
         serve_plain, summarize
     reqs = request_stream(requests, rate, seed)
@@ -529,10 +532,13 @@ def serve_form(a, hq, rate, requests, seed, plain=True):
     hc = out["hybrid_fp64"]["circuits_per_s"]
     out["speedup_vs_hybrid_fp64"] = round(out["engine_c64"]["circuits_per_s"] / hc, 2) if hc else None
     if plain:
-        circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in reqs]
-        precs, pwall = serve_plain(hq, reqs, circuits)
+        preqs = reqs[:plain_sample] if plain_sample > 0 else reqs
+        circuits = [build_circuit(hq, r["kind"], r["n"], r["depth"], r["seed"]) for r in preqs]
+        precs, pwall = serve_plain(hq, preqs, circuits)
         _progress(f"serve: rate {rate} plain done in {pwall:.1f} s")
         out["plain_fp64"] = summarize(precs, pwall)
+        if len(preqs) < len(reqs):
+            out["plain_fp64"]["sample"] = f"first {len(preqs)} of the {len(reqs)} requests"
         pc = out["plain_fp64"]["circuits_per_s"]
         out["speedup_vs_plain_fp64"] = round(out["engine_c64"]["circuits_per_s"] / pc, 2) if pc else None
     return out
