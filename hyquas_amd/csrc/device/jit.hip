@@ -663,10 +663,22 @@ int helper_compile(const std::string& src, std::vector<char>* code, std::string*
 {
     const std::string& h = helper_path();
     if (h.empty() || g_helper_failed.load()) return 0;
+    // sources and code objects go through a private directory (mode 0700,
+    // made once per process under TMPDIR), removed at exit
+    static const std::string dir = [] {
+        const char* t = getenv("TMPDIR");
+        std::string tmpl = std::string(t && *t ? t : "/tmp") + "/hq_rtc_XXXXXX";
+        std::vector<char> buf(tmpl.begin(), tmpl.end());
+        buf.push_back('\0');
+        if (!mkdtemp(buf.data())) return std::string();
+        static std::string made;
+        made = buf.data();
+        atexit([] { (void)rmdir(made.c_str()); });
+        return made;
+    }();
+    if (dir.empty()) return 0;
     static std::atomic<unsigned long long> counter{0};
-    const char* t = getenv("TMPDIR");
-    const std::string base = std::string(t && *t ? t : "/tmp") + "/hq_rtc_" + std::to_string((long long)getpid()) +
-                             "_" + std::to_string(counter++);
+    const std::string base = dir + "/k" + std::to_string(counter++);
     const std::string fin = base + ".hip", fout = base + ".co", flog = base + ".log";
     if (!write_file(fin, src.data(), src.size())) return 0;
     auto cleanup = [&] {

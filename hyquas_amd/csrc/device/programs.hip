@@ -314,16 +314,17 @@ static hq_status time_variants(hq_state s, const std::vector<std::vector<hq_prog
 hq_status build_sharemem_programs(hq_state s, const std::vector<SharememRequest>& reqs, int reg_qubits, bool jit,
                                   std::vector<hq_program>* out, int tune_mode)
 {
-    // tune_mode 1: four variants of build_sharemem_program, timed: plain
-    // stages; whole-line stages with non-temporal loads and stores; each with
-    // non-temporal stores only.  (All eight combinations, non-temporal loads
-    // alone included, tuned worse: 0.1574-0.1584 s against 0.1558 s at 30
-    // qubits; two launches per candidate cannot tell eight apart.)  2: the
-    // whole-line variant with non-temporal loads and stores only
+    // tune_mode 1: two variants of build_sharemem_program, timed: the plain
+    // stages, and the whole-line stages with non-temporal loads and stores.
+    // (Four candidates, adding each with non-temporal stores only, averaged
+    // 0.1575 s per 30-qubit simulation over five boxes against 0.1567 s for
+    // two; all eight combinations 0.1579 s: with two timed launches per
+    // candidate, more close candidates let the noise pick losers.)  2: the
+    // whole-line variant only
     bool tune = tune_mode == 1 && jit && !reqs.empty();
     const int only = tune_mode == 2 && jit ? 7 : 0;
-    static constexpr int kTuned[] = {0, 7, 2, 3};
-    const int variants = tune ? 4 : 1;
+    static constexpr int kTuned[] = {0, 7};
+    const int variants = tune ? 2 : 1;
     std::vector<std::vector<hq_program>> built(variants, std::vector<hq_program>(reqs.size(), nullptr));
     auto destroy_all = [&]() {
         for (auto& v : built)

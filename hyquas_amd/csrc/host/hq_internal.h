@@ -20,11 +20,10 @@ hq_status build_sharemem_program(hq_state s, const hq_gate* gates, int num_gates
 
 // Many OShareMem programs at once; with jit, every group also gets a
 // structure-specialised kernel, compiled in parallel.  tune_mode (jit only):
-// 1 compiles four program variants of every group (plain or whole-line
-// stages, non-temporal loads and stores or stores only), times them on the
-// state (its contents are overwritten) and keeps the fastest; 2 uses the
-// whole-line variant with non-temporal loads and stores for every group; 0
-// the plain one.
+// 1 compiles two program variants of every group (plain stages; whole-line
+// stages with non-temporal loads and stores), times them on the state (its
+// contents are overwritten) and keeps the faster; 2 uses the whole-line
+// variant for every group; 0 the plain one.
 struct SharememRequest {
     std::vector<hq_gate> gates;
     uint64_t mask = 0;

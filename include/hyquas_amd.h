@@ -189,9 +189,9 @@ hq_status hq_program_is_specialized(hq_program p, int *yes);
  * register and thread qubit counts.  variant bit 0: 0 is the plain kernel,
  * 1 the whole-line non-temporal one (stages without gates added where the
  * first or last stage would access HBM in pieces shorter than 128 bytes);
- * bit 5: non-temporal stores only (loads through the caches), on the plain
- * or the whole-line stages; these are the four kernels the "tune" option
- * chooses between; bit 1: the kernel of a
+ * the two kernels the "tune" option chooses between; bit 5: non-temporal
+ * stores only (loads through the caches), on the plain or the whole-line
+ * stages; bit 1: the kernel of a
  * run's first group, which builds |0...0> instead of loading the state;
  * bit 2: the batched form of the engine, hq_group_batch(items, count), whose
  * blocks map to (circuit, tile) through an item table (40-byte items:
@@ -279,10 +279,9 @@ hq_status hq_circuit_parse_qasm(const char *text, hq_circuit *out);
  *   "window"      gates considered per group (default 512)
  *   "jit"         specialised OShareMem kernels: 0 off, 1 on, -1 auto
  *                 (default: on from 22 qubits)
- *   "tune"        per-group choice, timed on the state at compile, among
- *                 four specialised kernels (plain or whole-line stages, each
- *                 with non-temporal loads and stores or non-temporal stores
- *                 only): 0 off, 1 on, 2 the whole-line kernel everywhere
+ *   "tune"        per-group choice, timed on the state at compile, between
+ *                 the plain specialised kernel and a whole-line non-temporal
+ *                 one: 0 off, 1 on, 2 the whole-line kernel everywhere
  *                 (untimed), -1 auto (default: on with specialised kernels
  *                 from 26 qubits); timing overwrites the state
  *   "transmm_inplace" TransMM groups: 1 (default) planned and run as one
