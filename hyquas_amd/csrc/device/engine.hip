@@ -793,8 +793,10 @@ static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int preci
             rc = join();
             if (rc != HQ_OK) return rc;
         }
+        // (build_round lists the group launches first; one after the join
+        // would run on the engine stream, after everything before it)
         hipStream_t st = main_st;
-        if (group && fork) st = side->streams[next_side++ % nside];
+        if (group && !joined) st = side->streams[next_side++ % nside];
         if (d.kind == 0) {
             rc = precision == HQ_C128
                      ? launch_batch_shape<double>(d.rb, d.tb, (const BatchItem*)tab, d.count, d.blocks, st)
