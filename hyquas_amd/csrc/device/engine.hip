@@ -365,13 +365,19 @@ struct SideStreams {
     hq_status create(int n)
     {
         if (!streams.empty() || n <= 1) return HQ_OK;
-        HQ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        if (!fork) HQ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         for (int i = 0; i < n; i++) {
-            hipStream_t st = nullptr;
-            HQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));   // default priority
-            streams.push_back(st);
+            // a stream and its join event enter together (or neither does)
             hipEvent_t ev = nullptr;
             HQ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            hipStream_t st = nullptr;
+            const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);   // default priority
+            if (e != hipSuccess) {
+                (void)hipEventDestroy(ev);
+                set_error(std::string("engine: side stream: ") + hipGetErrorString(e));
+                return HQ_ERR_HIP;
+            }
+            streams.push_back(st);
             join.push_back(ev);
         }
         return HQ_OK;
