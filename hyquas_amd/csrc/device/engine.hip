@@ -790,7 +790,7 @@ static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int preci
         for (size_t k = 0; k < nside; k++) HQ_HIP(hipStreamWaitEvent(side->streams[k], side->fork, 0));
     }
     size_t next_side = 0;
-    for (const LaunchDesc& d : rb.launches) {
+    auto launch = [&](const LaunchDesc& d) -> hq_status {
         const void* tab = (const void*)(uintptr_t)(table_dev + d.table_off);
         HQ_CHECK(d.blocks < (1ull << 31), HQ_ERR_UNSUPPORTED, "engine: launch too large");
         hq_status rc = HQ_OK;
@@ -824,7 +824,15 @@ static hq_status issue_round(const RoundBuild& rb, uint64_t table_dev, int preci
                                    d.count);
             HQ_HIP(hipGetLastError());
         }
-        if (rc != HQ_OK) return rc;
+        return rc;
+    };
+    for (const LaunchDesc& d : rb.launches) {
+        const hq_status rc = launch(d);
+        if (rc != HQ_OK) {
+            // the engine stream still waits for what the side streams hold
+            if (!joined) (void)join();
+            return rc;
+        }
     }
     if (!joined) return join();
     return HQ_OK;
