@@ -63,6 +63,22 @@ def _traffic_from_profiles(config, alg_bytes):
     return d.get("bytes_per_launch")
 
 
+def _batch_valu_from_profiles(name, config, launches):
+    """VALU wave-instructions of one batch replay measured by rocprofv3
+    (SQ_INSTS_VALU summed over the replay's dispatches), if a summary of
+    exactly this batch is committed under profiles/: same circuits, kernels
+    and tiles, and the same launches per replay; (count, peak rate) or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "batch_valu_hea4096.json")) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = d.get(name)
+    if d.get("config") != config or not rec or rec.get("launches_per_replay") != launches:
+        return None
+    return rec["valu_wave_instr_per_replay"], d["peak_wave_instr_per_s"]
+
+
 def cpu_baseline(n, layers, seed, gates_sample):
     import numpy as np
 
@@ -449,6 +465,15 @@ def batch_form(a, hq, count, plain_sample, precisions=("c64", "c128")):
                                   "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBS, 4),
                                   "pass_bytes": info["pass_bytes"], "device_ms_mean": round(dev_ms, 3)},
                      "max_norm_error": float(np.max(np.abs(ex[:, 0] - 1)))}
+        # the same replay against the VALU issue rate (instruction count from
+        # the committed counter profile when this is the profiled batch)
+        valu = _batch_valu_from_profiles(name, {"count": count, "seed": a.seed, "batch_jit": a.batch_jit,
+                                                "batch_tile": a.batch_tile}, info["launches"])
+        if valu:
+            rate = valu[0] / (dev_ms * 1e-3)
+            res[name]["valu_roofline"] = {"bound": "valu issue", "achieved": round(rate / 1e9, 1),
+                                          "peak": round(valu[1] / 1e9, 1), "unit": "G wave-instr/s",
+                                          "frac": round(rate / valu[1], 4), "wave_instr_per_replay": valu[0]}
         b.close()
         e.close()
         _progress(f"batch: {name} {res[name]['circuits_per_s']} circuits/s")
