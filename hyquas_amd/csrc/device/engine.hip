@@ -473,8 +473,10 @@ static int tile_for(const EngineOptions& o, int n)
     return std::min(k, n);
 }
 
-static void append_blob(JobPlan* jp, const ShmPlan& plan)
+static hq_status append_blob(JobPlan* jp, const ShmPlan& plan)
 {
+    for (const ShmOp& op : plan.ops)
+        HQ_CHECK(op.cls <= SHM_SCALE, HQ_ERR_UNSUPPORTED, "engine: op class the interpreter does not implement");
     GroupPlan g;
     g.rb = plan.rb;
     g.tb = plan.tb;
@@ -491,6 +493,7 @@ static void append_blob(JobPlan* jp, const ShmPlan& plan)
                plan.ops.size() * sizeof(ShmOp));
     g.blob_off = off;
     jp->groups.push_back(g);
+    return HQ_OK;
 }
 
 // A group run by a specialised batched kernel: its coefficients (state precision)
@@ -541,7 +544,16 @@ static hq_status add_group(JobPlan* jp, int n, int precision, const std::vector<
         }
         // the interpreter takes the group when its kernel cannot be built
     }
-    append_blob(jp, plan);
+    if (kc) {
+        // in the interpreter's op forms (the shared-code forms are the
+        // specialised kernels' own)
+        plan = ShmPlan();
+        st = plan_sharemem(n, precision, hg.data(), (int)hg.size(), mask, 0, &plan, false, 0, false);
+        if (st != HQ_OK) return st;
+        plan.hdr.zero_mask = zero_mask;
+    }
+    st = append_blob(jp, plan);
+    if (st != HQ_OK) return st;
     jp->groups.back().live = live;
     return HQ_OK;
 }

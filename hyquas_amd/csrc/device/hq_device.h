@@ -84,6 +84,11 @@ enum ShmClass : uint8_t {
     SHM_RXA = 10,   // lo' = lo + i t hi, hi' = i t lo + hi       (RX / cos)
     SHM_RXB = 11,   // lo' = u lo + i hi, hi' = i lo + u hi       (RX / sin)
     SHM_SCALE = 12, // every amplitude *= (m[0], m[1])
+    // rotations in specialised kernels shared by same-structure groups (one
+    // form for every angle; the scalar factor goes to SHM_SCALE): three
+    // shears, a = m[0], b = m[2], |a|, |b| <= 1
+    SHM_SHEAR = 13,  // lo += a hi, hi += b lo, lo += a hi        (RY)
+    SHM_SHEARX = 14, // lo += i a hi, hi -= i b lo, lo += i a hi  (RX)
 };
 
 enum ShmWhere : uint8_t { SHM_IN_REG = 0, SHM_IN_THREAD = 1, SHM_IN_BLOCK = 2 };

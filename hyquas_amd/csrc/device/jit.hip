@@ -270,6 +270,10 @@ struct Emitter {
             }
         }
         const bool rename = identity_ok_renaming && !blockc;
+        // the shear forms come from uncontrolled gates only; a per-lane
+        // identity select would not express the identity for them
+        if (op.cthr && (op.cls == SHM_SHEAR || op.cls == SHM_SHEARX))
+            throw std::runtime_error("shear op with a thread-bit control");
         if (op.cls == SHM_SCALE) {
             for (int j = 0; j < NR; j++) scale_stmt(j, mv[0], mv[1]);
         } else if (op.where == SHM_IN_REG) {
@@ -335,6 +339,20 @@ struct Emitter {
                     o << "      { const C lo = " << r(j) << ", hi = " << r(jh) << "; " << r(j) << ".x = " << mv[0]
                       << " * lo.x - hi.y; " << r(j) << ".y = " << mv[0] << " * lo.y + hi.x; " << r(jh) << ".x = "
                       << mv[0] << " * hi.x - lo.y; " << r(jh) << ".y = " << mv[0] << " * hi.y + lo.x; }\n";
+                    break;
+                case SHM_SHEAR:
+                    o << "      { C lo = " << r(j) << ", hi = " << r(jh) << "; "
+                      << "lo.x += " << mv[0] << " * hi.x; lo.y += " << mv[0] << " * hi.y; "
+                      << "hi.x += " << mv[2] << " * lo.x; hi.y += " << mv[2] << " * lo.y; "
+                      << "lo.x += " << mv[0] << " * hi.x; lo.y += " << mv[0] << " * hi.y; "
+                      << r(j) << " = lo; " << r(jh) << " = hi; }\n";
+                    break;
+                case SHM_SHEARX:
+                    o << "      { C lo = " << r(j) << ", hi = " << r(jh) << "; "
+                      << "lo.x -= " << mv[0] << " * hi.y; lo.y += " << mv[0] << " * hi.x; "
+                      << "hi.x += " << mv[2] << " * lo.y; hi.y -= " << mv[2] << " * lo.x; "
+                      << "lo.x -= " << mv[0] << " * hi.y; lo.y += " << mv[0] << " * hi.x; "
+                      << r(j) << " = lo; " << r(jh) << " = hi; }\n";
                     break;
                 default:  // SHM_PHASE
                     scale_stmt(jh, mv[6], mv[7]);

@@ -105,19 +105,56 @@ struct Lowered {
     double m[8];
 };
 
+// Rotation [[c, -s], [s, c]] (c >= 0 after the sign is moved into the
+// scalar) as three shears: [[1, a], [0, 1]] [[1, 0], [b, 1]] [[1, a], [0, 1]]
+// with a = -tan(phi / 2) = -s / (1 + c), b = s = sin(phi); |a|, |b| <= 1 and
+// no division by a small number, for every angle.
+void shear_form(double c, double s, double* a, double* b, std::complex<double>* S)
+{
+    const double r = std::hypot(c, s);
+    c /= r;
+    s /= r;
+    double sigma = 1.0;
+    if (c < 0) { c = -c; s = -s; sigma = -1.0; }
+    *S *= sigma * r;
+    *a = -s / (1.0 + c);
+    *b = s;
+}
+
 // Factor a scalar out of an uncontrolled gate so that its pair update needs
 // half the multiplies: H = r [[1, 1], [1, -1]]; a rotation [[c, -s], [s, c]]
 // = c [[1, -t], [t, 1]] (t = s/c) or s [[u, -1], [1, u]] (u = c/s), whichever
 // keeps |t|, |u| <= 1; RX-type [[a, ib], [ib, a]] likewise; a unit diagonal
 // diag(d0, d1) = d0 diag(1, d1/d0).  Other gates are left as they are.
-void factor_scalar(Lowered& L, std::complex<double>* S, bool shared_code)
+// True when the gate was rewritten (its scalar multiplied into S).
+bool factor_scalar(Lowered& L, std::complex<double>* S, bool shared_code)
 {
     double* m = L.m;
     // shared_code: only factorings whose op class does not depend on the
-    // angle (H, unit diagonals), so groups that differ only in angles emit the
-    // same specialised kernel source
+    // angle, so groups that differ only in angles emit the same specialised
+    // kernel source: H, unit diagonals, and rotations as three shears
+    if (shared_code && L.cls == SHM_REAL && m[0] == m[6] && m[2] == -m[4] && (m[0] != 0 || m[4] != 0)) {
+        double a, b;
+        shear_form(m[0], m[4], &a, &b, S);
+        L.cls = SHM_SHEAR;
+        memset(m, 0, sizeof(L.m));
+        m[0] = a;
+        m[2] = b;
+        return true;
+    }
+    if (shared_code && L.cls == SHM_RX && m[0] == m[6] && m[3] == m[5] && (m[0] != 0 || m[3] != 0)) {
+        // [[c, -i s], [-i s, c]] = diag(1, -i) [[c, -s], [s, c]] diag(1, i)
+        double a, b;
+        shear_form(m[0], -m[3], &a, &b, S);
+        L.cls = SHM_SHEARX;
+        memset(m, 0, sizeof(L.m));
+        m[0] = a;
+        m[2] = b;
+        return true;
+    }
     if (shared_code && (L.cls == SHM_RX || (L.cls == SHM_REAL && !(m[0] == m[2] && m[0] == m[4] && m[0] == -m[6]))))
-        return;
+        return false;
+    const int cls_in = L.cls;
     if (L.cls == SHM_REAL) {
         if (m[0] == m[2] && m[0] == m[4] && m[0] == -m[6] && m[0] != 0) {
             *S *= m[0];
@@ -147,6 +184,7 @@ void factor_scalar(Lowered& L, std::complex<double>* S, bool shared_code)
             m[7] = q.imag();
         }
     }
+    return L.cls != cls_in;
 }
 
 // swizzle column of tile bit i (bits >= 3 fold into the low 3 bits)
@@ -201,6 +239,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
     // product S is applied once at the end of the pass (HQ_SHM_FACTOR=0: off)
     static const bool factor = !getenv("HQ_SHM_FACTOR") || atoi(getenv("HQ_SHM_FACTOR")) != 0;
     std::complex<double> S(1.0, 0.0);
+    bool factored = false;
     for (int i = 0; i < num_gates; i++) {
         const hq_gate& g = gates[i];
         hq_status st = validate_gate(g, n);
@@ -221,7 +260,7 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
             L.c_block[c] = cs[c] >= 0 ? block_of[cs[c]] : -1;
         }
         memcpy(L.m, g.mat, sizeof(L.m));
-        if (factor && g.control < 0 && g.control2 < 0) factor_scalar(L, &S, shared_code);
+        if (factor && g.control < 0 && g.control2 < 0 && factor_scalar(L, &S, shared_code)) factored = true;
         low.push_back(L);
         low_type.push_back(g.type);
     }
@@ -377,7 +416,9 @@ hq_status plan_sharemem(int n, int precision, const hq_gate* gates, int num_gate
         d.op_end = (uint32_t)ops.size();
         dst.push_back(d);
     }
-    if (S != std::complex<double>(1.0, 0.0)) {
+    // shared code: the scale op whenever a gate was factored (a structural
+    // fact), even where the angles make the product exactly 1
+    if (S != std::complex<double>(1.0, 0.0) || (shared_code && factored)) {
         ShmOp o;
         memset(&o, 0, sizeof(o));
         o.cls = SHM_SCALE;

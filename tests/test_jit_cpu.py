@@ -2,6 +2,7 @@
 library emits for a group is compiled with g++ and executed with one host
 thread per GPU thread (tests/cpp/jit_harness.cpp), then compared with the FP64
 oracle.  No GPU involved."""
+import math
 import os
 import shutil
 import subprocess
@@ -169,6 +170,59 @@ def test_batched_kernel_matches_oracle(hq, tmp_path, n, k, seed, prec, rb, varia
         same = np.fromfile(tmp_path / "out2.bin", dtype=dt).reshape(3, -1)
         assert np.array_equal(same, np.concatenate(psi0).astype(dt).reshape(3, -1))
         assert np.array_equal(np.fromfile(tmp_path / "part2.bin").reshape(3, -1, 3), part)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batched_rotations_as_shears(hq, tmp_path, prec):
+    """Shared-code kernels apply RY and RX as three shears whose scalar goes
+    to the final scale op: one source for every angle, including the ones
+    where the rotation's cosine is negative (|theta| > pi: the sign moves
+    into the scalar), pi and 2 pi, and angles near zero (exactly zero makes
+    the gate diagonal, another structure); each circuit's state equal to the
+    oracle's."""
+    from helpers import CZ, H, RX, RY, RZ
+    n, k = 12, 10
+    mask = (1 << k) - 1
+    angle_sets = [
+        [3.5, -3.9, 6.0, -6.2, 4.4, 0.3],
+        [math.pi, -math.pi, 2 * math.pi, -2 * math.pi, 1e-12, 1e-9],
+        [0.1, -0.7, 1.3, 2.9, -3.1, 3.1416],
+    ]
+    specs = []
+    for ang in angle_sets:
+        sp = [(H, 0, -1, ())]
+        for i, th in enumerate(ang):
+            sp.append((RY if i % 2 == 0 else RX, i % k, -1, (th,)))
+            sp.append((RZ, (i + 3) % k, -1, (0.5 * th + 0.1,)))
+            sp.append((CZ, (i + 1) % k, i % k, ()))
+        sp += [(RY, 9, -1, (ang[0],)), (RX, 8, -1, (ang[1],))]
+        specs.append(sp)
+    srcs, coefs = [], []
+    for sp in specs:
+        src, coef, (tile, rbits, tbits) = hq.sharemem_source(n, to_oracle(sp), mask, prec, 0, 4 | 16)
+        srcs.append(src)
+        coefs.append(np.asarray(coef, dtype=np.float64))
+    assert srcs[0] == srcs[1] == srcs[2], "rotations at any angle must give one kernel source"
+    assert "lo.x += " in srcs[0] and "lo.x -= " in srcs[0], "RY and RX as shears"
+    cxx = CLANG if os.path.exists(CLANG) else "g++"
+    if cxx == "g++" and "ext_vector_type" in srcs[0]:
+        pytest.skip("needs ROCm clang++ for clang vector extensions")
+    ksrc = tmp_path / "kernel.hip"
+    ksrc.write_text(srcs[0])
+    exe = tmp_path / "harness"
+    subprocess.run([cxx, "-O1", "-std=c++20", "-pthread", "-Wno-unknown-attributes", f'-DGROUP_SRC="{ksrc}"',
+                    os.path.join(HERE, "cpp", "jit_batch_harness.cpp"), "-o", str(exe)], check=True)
+    dt = np.complex128 if prec == 0 else np.complex64
+    psi0 = [_rand_state(n, 70 + i) for i in range(3)]
+    np.concatenate(psi0).astype(dt).tofile(tmp_path / "in.bin")
+    np.concatenate(coefs).tofile(tmp_path / "coef.bin")
+    subprocess.run([str(exe), str(n), str(k), str(1 << tbits), "3", "0", str(tmp_path / "in.bin"),
+                    str(tmp_path / "coef.bin"), str(tmp_path / "out.bin"), str(tmp_path / "part.bin")],
+                   check=True, timeout=300)
+    got = np.fromfile(tmp_path / "out.bin", dtype=dt).reshape(3, -1)
+    for i in range(3):
+        want = oracle.run_plain(to_oracle(specs[i]), n, psi0[i].astype(dt).astype(np.complex128))
+        assert rel_err(got[i], want) <= (1e-13 if prec == 0 else 2e-6)
 
 
 def rand_state_or_zero(n, seed, init):
