@@ -199,7 +199,8 @@ hq_status hq_program_is_specialized(hq_program p, int *yes);
  * store); bit 3 (with bit 2): the circuit's last group, which also writes
  * each tile's sum p, sum p Z and sum p ZZ to the item's partials; bit 4:
  * op classes independent of the gate angles (same-structure groups share
- * one source).  For inspection, caching and CPU tests. */
+ * one source; RY and RX as three shears, their scalars in the final scale
+ * op).  For inspection, caching and CPU tests. */
 hq_status hq_sharemem_source(int num_qubits, int precision, const hq_gate *gates, int num_gates,
                              uint64_t shared_mask, int reg_qubits, int variant, char *buf, size_t cap,
                              size_t *len, double *coef, size_t coef_cap, size_t *num_coef, int *shape);
